@@ -1,0 +1,165 @@
+"""Throughput benchmark of the D3Q27 colour-gradient step on one MI355X.
+
+Workload (default): the round jet on 256x512x256 nodes (ghost layers
+included), inlet disk R=16 lu, U=0.05, tau_r/tau_b = 0.55/0.6 (the reference
+relaxes with tau_b everywhere), A=1e-3, beta=0.7, FP32 populations, radial
+statistics sampled every 10 steps.  Synthetic initial state: the reference's
+jet initialisation (jet/jetIn.cuh).  A step is one full time step (moments,
+colour gradient, collision, perturbation, recolouring, streaming, inlet,
+outlet, statistics when due) over the whole grid, state resident in HBM.
+
+MLUPS follows utilities/countmlups.cuh:42-48 (steps * NX*NY*NZ / t, ghost
+layers counted); the interior-only figure is reported too.  The roofline
+uses 432 algorithmic bytes per interior lattice update (27 populations x 2
+colours read + written, FP32) against 8 TB/s.
+
+The CPU baseline runs the FP64 OpenMP oracle (oracle/, a restatement of the
+reference step) on the same grid for a bounded number of steps.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload jet256|jet128|bubble64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_LU = {"fp32": 432, "fp16": 216, "bf16": 216}
+
+
+def make_case(name: str):
+    import multicomponent_color_gradient_amd as m
+
+    if name == "jet256":
+        c = m.jet_case(256, 512, 256, jet_radius=16.0, jet_velocity=0.05)
+    elif name == "jet128":
+        c = m.jet_case(128, 256, 128, jet_radius=8.0, jet_velocity=0.05)
+    elif name == "bubble64":
+        c = m.periodic_bubble_case(64, radius=12.0, tau=0.8, A=1e-3, beta=0.7)
+        return c, "bubble_tanh"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    c.taur, c.taub = 0.55, 0.6
+    c.A_override = 1e-3
+    c.beta_recolor = 0.7
+    c.NSTATS_START_STEP = 0
+    c.NSTATS_SAMPLE = 10
+    return c, "jet"
+
+
+def load_traffic(workload: str, storage: str):
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}_{storage}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        d = json.load(fh)
+    return d.get("bytes_per_launch")
+
+
+def cpu_baseline(case, init: str, budget_s: float):
+    """FP64 OpenMP oracle on the same grid: 1 warm-up step, then steps until budget_s."""
+    from oracle import oracle as orc
+
+    t0 = time.perf_counter()
+    o = orc.Oracle(case)
+    getattr(o, "init_" + init)()
+    o.step(1)
+    t_setup = time.perf_counter() - t0
+    n = 0
+    t1 = time.perf_counter()
+    while True:
+        o.step(1)
+        n += 1
+        if time.perf_counter() - t1 >= budget_s or n >= 50:
+            break
+    dt = time.perf_counter() - t1
+    cells = case.NX * case.NY * case.NZ
+    o.close()
+    return {
+        "value": round(n * cells / dt / 1e6, 3),
+        "unit": "MLUPS",
+        "cores": orc.num_threads(),
+        "kind": "port",
+        "sample": f"FP64 OpenMP oracle, {case.NX}x{case.NY}x{case.NZ}, 1 warm-up + {n} timed steps "
+                  f"({dt:.1f} s, setup {t_setup:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="jet256")
+    ap.add_argument("--storage", default="fp32")
+    ap.add_argument("--kernel", default="fused")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this solver runs on one GPU; multi-GPU runs are not part of this build "
+              "(a y-slab decomposition is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import multicomponent_color_gradient_amd as m
+
+    case, init = make_case(args.workload)
+    s = m.Solver(case, storage=args.storage, kernel=args.kernel)
+    getattr(s, "init_" + init)()
+    print(f"bench: {args.workload} {case.NX}x{case.NY}x{case.NZ} storage={args.storage} kernel={args.kernel} "
+          f"warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+    s.step(args.warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    ms = s.time_steps(args.steps)
+    wall = time.perf_counter() - t0
+    s.check_health()
+    cells = case.NX * case.NY * case.NZ
+    interior = m.interior_nodes(case)
+    sec = ms / 1e3
+    mlups = args.steps * cells / sec / 1e6
+    mlups_int = args.steps * interior / sec / 1e6
+    avg_ms = ms / args.steps
+    bpl = BYTES_PER_LU[args.storage]
+    achieved = bpl * interior / (avg_ms / 1e3) / 1e9
+    traffic = load_traffic(args.workload, args.storage)
+    s.close()
+
+    out = {
+        "metric": "MLUPS (fused step, % HBM roofline at 432 B/LU FP32)",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(avg_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.storage == "fp32" else f"{args.storage}-storage/f32-math",
+        "data": "synthetic (reference jet initialisation, sharp inlet)",
+        "config": {"workload": f"round jet {case.NX}x{case.NY}x{case.NZ}, R={case.jet_radius:g}, U={case.jet_velocity:g}"
+                               if init == "jet" else f"periodic tanh bubble {case.NX}^3",
+                   "NX": case.NX, "NY": case.NY, "NZ": case.NZ, "storage": args.storage, "kernel": args.kernel,
+                   "stats_every": case.NSTATS_SAMPLE, "mlups_interior": round(mlups_int, 2),
+                   "wall_s": round(wall, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_lu": bpl, "kernel_ms": round(avg_ms, 5)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, init, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

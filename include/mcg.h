@@ -1,0 +1,232 @@
+/*
+ * mcg.h -- C ABI of the MI355X D3Q27 color-gradient lattice Boltzmann solver.
+ *
+ * The reference (Felipe-Aristi/Multicomponent_Color_Gradient) has no plugin or
+ * FFI layer: its operator API is the set of inline host launchers in
+ * launch.cuh:13-119 plus the allocation helpers in memory.cuh:58-133 and
+ * meanF/memoryMeanFields.cuh:57-126, driven by main.cu:17-120.  Every entry
+ * point below names the reference function(s) it replaces.
+ *
+ * Conventions
+ *   - Every function returns MCG_OK (0) or a negative MCG_E* status; the
+ *     library never exits the process (the reference's CUDA_CHECK calls
+ *     std::exit, utilities/cudaUtilities.cuh:9-19).  mcg_last_error() returns
+ *     a thread-local message for the last failure.
+ *   - Grid indexing is the reference's: cell id = x + NX*(y + NY*z)
+ *     (utilities/indexing.cuh:10-13), population id = i*Ncells + id
+ *     (utilities/indexing.cuh:15-19), D3Q27 directions in the order of
+ *     stencil_ct.cuh:9-16.  Host buffers passed to the copy functions use
+ *     this layout and cover the full NX*NY*NZ grid, ghosts included.
+ *   - One solver handle = one device, one HIP stream (hipStream_t passed as
+ *     void*, NULL = the null stream).  A handle is not thread safe.
+ */
+#ifndef MCG_H
+#define MCG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCG_OK 0
+#define MCG_E_INVALID (-1)     /* bad argument or case                         */
+#define MCG_E_HIP (-2)         /* HIP runtime error                            */
+#define MCG_E_NOMEM (-3)       /* device or host allocation failed             */
+#define MCG_E_STATE (-4)       /* call out of order (e.g. step before init)    */
+#define MCG_E_UNSUPPORTED (-5) /* option combination not implemented           */
+#define MCG_E_NUMERIC (-6)     /* watchdog: NaN/Inf or non-positive density    */
+#define MCG_E_IO (-7)          /* file could not be written                    */
+
+/* y-direction treatment */
+#define MCG_Y_JET 0      /* inlet at y=0, convective outlet at y=NY-1 (jet/boundary_conditions.cuh) */
+#define MCG_Y_PERIODIC 1 /* periodic in y with the reference's wrap (utilities/bounds.cuh:36-47)      */
+
+/* collision relaxation time */
+#define MCG_VISC_SINGLE 0    /* omega = 1/taub everywhere (lbm.cuh:211, forces.cuh:102)             */
+#define MCG_VISC_INTERFACE 1 /* tau = (1+psi)/2 taur + (1-psi)/2 taub per node (forces.cuh:77-85)   */
+
+/* inlet velocity profile (MCG_Y_JET only) */
+#define MCG_INLET_SHARP 0 /* top-hat disk, jet/boundary_conditions.cuh:97-111                        */
+#define MCG_INLET_TANH 1  /* tanh-edged top-hat plus seeded transverse perturbation (extension)       */
+
+/* population storage */
+#define MCG_STORE_FP32 0
+#define MCG_STORE_FP16 1 /* fp16 offsets from the lattice weights, fp32 arithmetic */
+#define MCG_STORE_BF16 2 /* bf16 offsets from the lattice weights, fp32 arithmetic */
+
+/* step kernel variant */
+#define MCG_KERNEL_FUSED 0   /* one fused launch per step (default)                    */
+#define MCG_KERNEL_TWOPASS 1 /* moments pass + collide pass (the reference's data flow) */
+
+/*
+ * Case description: the run-time form of the compile-time constants of
+ * constants.cuh:8-76.  Names follow constants.cuh.  Derived quantities
+ * (nu, omega, sigma_over_4cs4) are computed from these fields exactly as
+ * constants.cuh:59-76 does; mcg_case_reference_jet() fills the reference's
+ * values.
+ */
+typedef struct mcg_case {
+    int32_t NX, NY, NZ;          /* constants.cuh:14-16 (ghost layers included)  */
+    int32_t NSTEP;               /* constants.cuh:8                              */
+    int32_t NOUTPUT;             /* constants.cuh:9                              */
+    int32_t NSTATS_START_STEP;   /* constants.cuh:10                             */
+    int32_t NSTATS_SAMPLE;       /* sampling period of the radial statistics;
+                                    the reference samples every step (main.cu:58-62,
+                                    meanF/meanFieldsRT.cuh:246 writes 1)         */
+    int32_t write_vti_output;    /* constants.cuh:11                             */
+    int32_t sponge_cells;        /* constants.cuh:17                             */
+    int32_t y_mode;              /* MCG_Y_*                                      */
+    int32_t wrap_gradient;       /* 0: colour gradient reads ghost nodes as they
+                                    are (forces.cuh:31-33); 1: periodic wrap     */
+    int32_t viscosity_mode;      /* MCG_VISC_*                                   */
+    int32_t inlet_profile;       /* MCG_INLET_*                                  */
+    int32_t perturbation_seed;   /* seed of the inlet perturbation (MCG_INLET_TANH) */
+    int32_t reserved_i[3];
+
+    double jet_radius, jet_x0, jet_z0, jet_velocity;       /* constants.cuh:32-35 */
+    double bubble_radius, bubble_x0, bubble_y0, bubble_z0; /* constants.cuh:38-41 */
+    double rhor0, rhob0;         /* constants.cuh:56-57                          */
+    double Re, We;               /* constants.cuh:59,71 (labels of output folders) */
+    double taur, taub;           /* constants.cuh:62-64                          */
+    double sigma;                /* constants.cuh:73                             */
+    double beta_recolor;         /* constants.cuh:74                             */
+    double A_override;           /* >0: surface-tension amplitude A used as given;
+                                    <=0: A = sigma/(4 cs^4 tau) (forces.cuh:87-90) */
+    double sponge_K;             /* utilities/mathUtilities.cuh:50 (reference 0)  */
+    double interface_width;      /* tanh width for mcg_init_bubble_tanh / inlet   */
+    double perturbation_amp;     /* inlet transverse perturbation, fraction of U  */
+    double reserved_d[4];
+} mcg_case;
+
+typedef struct mcg_solver mcg_solver;
+
+/* ---- case helpers ------------------------------------------------------ */
+
+/* Fill *c with the reference case of constants.cuh:8-76 (128x400x128 jet,
+ * Re 5000, We 2000). */
+int mcg_case_reference_jet(mcg_case *c);
+
+/* Validate a case; returns MCG_OK or MCG_E_INVALID / MCG_E_UNSUPPORTED. */
+int mcg_case_validate(const mcg_case *c);
+
+/* sizeof(mcg_case), for bindings that mirror the struct. */
+size_t mcg_case_size(void);
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* Allocate the device state for case *c on the current HIP device.
+ * Replaces allocate_device_memory (memory.cuh:58-99), allocate_mean_fields_device
+ * (meanF/memoryMeanFields.cuh:57-78) and make_cudaConfig (utilities/cudaConfig.cuh:22-31).
+ * storage: MCG_STORE_*.  stream: hipStream_t or NULL. */
+int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out);
+
+/* Free everything.  Replaces free_device_memory (memory.cuh:113-133) and
+ * free_mean_fields_device (meanF/memoryMeanFields.cuh:103-115). */
+int mcg_destroy(mcg_solver *s);
+
+/* Select the step kernel (MCG_KERNEL_*).  Both give identical results. */
+int mcg_set_kernel(mcg_solver *s, int kernel);
+
+/* ---- case API: initial conditions ------------------------------------- */
+
+/* Jet initial state: launch_jetDensity + launch_Injet (launch.cuh:19-29,
+ * main.cu:39-40; jet/jetIn.cuh:13-62).  Resets the step counter and the
+ * statistics. */
+int mcg_init_jet(mcg_solver *s);
+
+/* Sharp bubble initial state: launch_bubble (launch.cuh:13-17,
+ * bubble/bubbleIn.cuh:10-39). */
+int mcg_init_bubble(mcg_solver *s);
+
+/* Extension: bubble with a tanh interface, blue fraction
+ * (1 + tanh((R - r)/interface_width))/2, populations w_i * rho_k. */
+int mcg_init_bubble_tanh(mcg_solver *s);
+
+/* ---- the hot path ---------------------------------------------------- */
+
+/* Advance nsteps time steps.  One step is the body of the reference loop
+ * main.cu:54-66: launch_Macros -> [launch_accumulate_radial_moments once
+ * step >= NSTATS_START_STEP, every NSTATS_SAMPLE steps] -> launch_collistream
+ * -> launch_inlet_bc -> launch_neumann_bc (launch.cuh:31-119).  Asynchronous
+ * on the solver's stream. */
+int mcg_step(mcg_solver *s, int nsteps);
+
+/* Wait for the solver's stream. */
+int mcg_synchronize(mcg_solver *s);
+
+/* Number of steps taken since the last init. */
+int mcg_get_step(const mcg_solver *s, int64_t *step);
+
+/* ---- statistics and diagnostics ------------------------------------- */
+
+/* Radial statistics state (MeanFieldsState, meanF/memoryMeanFields.cuh:39-45). */
+int mcg_stats_info(const mcg_solver *s, int32_t *started, int64_t *start_step, int64_t *samples);
+
+/* Zero the radial accumulators and restart the start-step logic. */
+int mcg_stats_reset(mcg_solver *s);
+
+/* Copy the radial accumulators, each NY*NR_BINS (row y, column r-bin),
+ * NR_BINS = NX/2.  Replaces the D2H copies of write_radial_profile_outputs
+ * (meanF/meanFieldsRT.cuh:305-328).  sum_phi / sum_phi2 (phase field psi and
+ * its square) are an extension; any pointer may be NULL to skip it. */
+int mcg_copy_radial(mcg_solver *s, double *sum_uy, double *sum_uy2, double *sum_ur,
+                    double *sum_ur2, double *sum_uruy, uint64_t *count,
+                    double *sum_phi, double *sum_phi2);
+
+/* Total kinetic energy sum over interior nodes of 0.5|u|^2 (FP64).
+ * Replaces compute_total_tke (kernels.cu:136-152, launch.cuh:39-49). */
+int mcg_tke(mcg_solver *s, double *tke_total);
+
+/* FP64 global sums over interior nodes: out[0]=sum rho_r, out[1]=sum rho_b,
+ * out[2..4]=sum rho*u.  Extension (the reference has none). */
+int mcg_global_sums(mcg_solver *s, double out[5]);
+
+/* Watchdog: returns MCG_E_NUMERIC if any interior density is NaN/Inf or <= 0. */
+int mcg_check_health(mcg_solver *s);
+
+/* ---- state exchange --------------------------------------------------- */
+
+/* Copy rho_r, rho_b, u_x, u_y, u_z of the full grid (NX*NY*NZ each, ghosts
+ * included) as they stand after the last step: interior nodes hold the
+ * moments computed at the start of that step, ghost nodes the boundary values.
+ * Replaces copy_out_D2H (memory.cuh:135-142). */
+int mcg_copy_fields(mcg_solver *s, float *rhor, float *rhob, float *ux, float *uy, float *uz);
+
+/* Copy the populations f_r, f_b (27*NX*NY*NZ each, reference layout) that
+ * the next step will read; ghost-node slots are written as 0. */
+int mcg_copy_populations(mcg_solver *s, float *fr, float *fb);
+
+/* Overwrite the interior populations (reference layout, ghost slots ignored).
+ * Ghost densities and boundary state are left as the last init set them. */
+int mcg_set_populations(mcg_solver *s, const float *fr, const float *fb);
+
+/* ---- checkpoint / restart (extension) ------------------------------- */
+
+/* Size in bytes of a checkpoint image. */
+int mcg_checkpoint_size(const mcg_solver *s, uint64_t *nbytes);
+/* Copy the full device state (populations, boundary state, step counter,
+ * statistics) into host memory / restore it.  Bitwise round trip. */
+int mcg_checkpoint_save(mcg_solver *s, void *host_buf, uint64_t nbytes);
+int mcg_checkpoint_load(mcg_solver *s, const void *host_buf, uint64_t nbytes);
+
+/* ---- measurement ------------------------------------------------------ */
+
+/* Run nsteps steps bracketed by HIP events on the solver's stream and return
+ * the elapsed milliseconds (pure step launches, no host sync inside). */
+int mcg_time_steps(mcg_solver *s, int nsteps, double *ms);
+
+/* Bytes of device memory held by the solver. */
+int mcg_device_bytes(const mcg_solver *s, uint64_t *nbytes);
+
+/* ---- misc ------------------------------------------------------------- */
+
+const char *mcg_last_error(void);
+const char *mcg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCG_H */

@@ -1,0 +1,677 @@
+// api.hip -- C ABI of the solver (declared in include/mcg.h): device memory,
+// launch sequencing and state exchange.  One translation unit; the kernels
+// come from kernels.hpp (two-pass step, service kernels) and fused.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/mcg.h"
+#include "device.hpp"
+#include "fused.hpp"
+#include "kernels.hpp"
+
+using namespace mcg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return fail(MCG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+#define LAUNCH_CHECK(name)                                                                            \
+    do {                                                                                              \
+        hipError_t e_ = hipGetLastError();                                                            \
+        if (e_ != hipSuccess) return fail(MCG_E_HIP, std::string("launch ") + name + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kInitJet = 0, kInitBubble = 1, kInitBubbleTanh = 2;
+
+}  // namespace
+
+struct mcg_solver {
+    mcg_case c{};
+    DevParams P{};
+    int storage = MCG_STORE_FP32;
+    int kernel = MCG_KERNEL_FUSED;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    float *f[2] = {nullptr, nullptr};
+    float *bin[2] = {nullptr, nullptr};
+    float *bout[2] = {nullptr, nullptr};
+    float *gB[2] = {nullptr, nullptr};
+    float *grr = nullptr, *grb = nullptr;
+    float *mom = nullptr;
+    double *stats = nullptr;
+    unsigned long long *count = nullptr;
+    double *red_partials = nullptr;
+    double *red_out = nullptr;
+    int red_blocks = 0;
+    float *staging = nullptr;
+    size_t staging_bytes = 0;
+
+    int cur = 0;
+    int64_t step = 0;
+    bool initialized = false;
+    bool moments_valid = false;
+    bool inlet_reset_pending = false;
+    bool started = false;
+    int64_t start_step = 0, samples = 0;
+    InitGeo geo{};
+    size_t bytes = 0;
+
+    size_t N() const { return size_t(c.NX) * c.NY * c.NZ; }
+    size_t nxz() const { return size_t(c.NX) * c.NZ; }
+    size_t pop_bytes() const { return size_t(2 * Q) * P.plane * sizeof(float); }
+    size_t bnd_bytes() const { return size_t(18) * nxz() * sizeof(float); }
+    size_t gB_bytes() const { return size_t(5) * nxz() * sizeof(float); }
+    size_t bins() const { return size_t(c.NY) * P.NR; }
+
+    dim3 grid_interior() const { return dim3(unsigned(P.PX / 64), unsigned((P.IY + 3) / 4), unsigned(P.IZ)); }
+    dim3 grid_full() const { return dim3(unsigned((c.NX + 63) / 64), unsigned((c.NY + 3) / 4), unsigned(c.NZ)); }
+
+    DevBufs bufs(bool do_stats) const {
+        DevBufs B;
+        B.f_src = f[cur];
+        B.f_dst = f[cur ^ 1];
+        B.in_src = bin[cur];
+        B.in_dst = bin[cur ^ 1];
+        B.out_src = bout[cur];
+        B.out_dst = bout[cur ^ 1];
+        B.gB_src = gB[cur];
+        B.gB_dst = gB[cur ^ 1];
+        B.ghost_rr = grr;
+        B.ghost_rb = grb;
+        B.mom = mom;
+        B.stats = stats;
+        B.count = count;
+        B.do_stats = do_stats ? 1 : 0;
+        return B;
+    }
+
+    int ensure_staging(size_t nb) {
+        if (staging_bytes >= nb) return MCG_OK;
+        if (staging) (void)hipFree(staging);
+        staging = nullptr;
+        staging_bytes = 0;
+        if (hipMalloc(&staging, nb) != hipSuccess) return fail(MCG_E_NOMEM, "staging allocation failed");
+        staging_bytes = nb;
+        return MCG_OK;
+    }
+    void release_staging() {
+        if (staging) (void)hipFree(staging);
+        staging = nullptr;
+        staging_bytes = 0;
+    }
+};
+
+namespace {
+
+// derived constants of constants.cuh:59-76 for a case
+void fill_params(const mcg_case &c, int storage, DevParams &P) {
+    P.NX = c.NX;
+    P.NY = c.NY;
+    P.NZ = c.NZ;
+    P.PX = ((c.NX - 2) + 63) / 64 * 64;
+    P.IY = c.NY - 2;
+    P.IZ = c.NZ - 2;
+    P.plane = uint32_t(size_t(P.PX) * P.IY * P.IZ);
+    P.y_mode = c.y_mode;
+    P.wrap_gradient = c.wrap_gradient;
+    P.visc_mode = c.viscosity_mode;
+    P.sponge_yend = c.NY - 2;
+    P.sponge_ystart = P.sponge_yend - c.sponge_cells + 1;
+    const double cs2 = 1.0 / 3.0;
+    P.nu = float((c.taub - 0.5) * cs2);
+    P.sponge_K = float(c.sponge_K);
+    P.taur = float(c.taur);
+    P.taub = float(c.taub);
+    P.omega_b = 1.0f / float(c.taub);
+    P.A_fixed = c.A_override > 0 ? float(c.A_override) : 0.0f;
+    P.sig4 = float(c.sigma) / (4.0f * mcg::cs4);
+    P.beta = float(c.beta_recolor);
+    P.jet_x0 = float(c.jet_x0);
+    P.jet_z0 = float(c.jet_z0);
+    P.jet_R = float(c.jet_radius);
+    P.jet_U = float(c.jet_velocity);
+    P.rhor0 = float(c.rhor0);
+    P.rhob0 = float(c.rhob0);
+    P.NR = c.NX / 2;
+    P.storage = storage;
+}
+
+int check_solver(const mcg_solver *s) {
+    if (!s) return fail(MCG_E_INVALID, "null solver");
+    return MCG_OK;
+}
+
+int check_ready(const mcg_solver *s) {
+    if (!s) return fail(MCG_E_INVALID, "null solver");
+    if (!s->initialized) return fail(MCG_E_STATE, "solver not initialised (call mcg_init_jet / mcg_init_bubble first)");
+    return MCG_OK;
+}
+
+int launch_moments(mcg_solver *s, int which) {
+    hipLaunchKernelGGL(k_moments, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->f[which], s->bin[which],
+                       s->bout[which], s->mom);
+    LAUNCH_CHECK("k_moments");
+    return MCG_OK;
+}
+
+// interior moments of the last step (the fused kernel does not store them)
+int ensure_moments(mcg_solver *s) {
+    if (s->moments_valid) return MCG_OK;
+    const int rc = launch_moments(s, s->cur ^ 1);
+    if (rc) return rc;
+    s->moments_valid = true;
+    return MCG_OK;
+}
+
+int run_reduce(mcg_solver *s, double out[kRedQ]) {
+    const size_t total = size_t(s->c.NX - 2) * (s->c.NY - 2) * (s->c.NZ - 2);
+    const int npb = int((total + s->red_blocks - 1) / s->red_blocks);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(s->red_blocks), dim3(256), 0, s->stream, s->P, s->mom, s->red_partials, npb);
+    LAUNCH_CHECK("k_reduce_partials");
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s->stream, s->red_partials, s->red_blocks, s->red_out);
+    LAUNCH_CHECK("k_reduce_final");
+    HIP_TRY(hipMemcpyAsync(out, s->red_out, sizeof(double) * kRedQ, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MCG_OK;
+}
+
+int do_init(mcg_solver *s, int kind) {
+    if (!s) return fail(MCG_E_INVALID, "null solver");
+    const mcg_case &c = s->c;
+    s->geo.g[0] = float(c.bubble_x0);
+    s->geo.g[1] = float(c.bubble_y0);
+    s->geo.g[2] = float(c.bubble_z0);
+    s->geo.g[3] = float(c.bubble_radius);
+    s->geo.g[4] = float(c.interface_width);
+    if (kind == kInitBubbleTanh && !(c.interface_width > 0))
+        return fail(MCG_E_INVALID, "interface_width must be > 0 for the tanh bubble");
+    s->cur = 0;
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(hipMemsetAsync(s->bin[b], 0, s->bnd_bytes(), s->stream));
+        HIP_TRY(hipMemsetAsync(s->bout[b], 0, s->bnd_bytes(), s->stream));
+    }
+    HIP_TRY(hipMemsetAsync(s->mom, 0, size_t(11) * s->P.plane * sizeof(float), s->stream));
+    hipLaunchKernelGGL(k_init_pull, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, kind, s->geo, s->f[0], s->bin[0],
+                       s->bout[0]);
+    LAUNCH_CHECK("k_init_pull");
+    hipLaunchKernelGGL(k_init_ghost, s->grid_full(), dim3(64, 4, 1), 0, s->stream, s->P, kind, s->geo, s->grr, s->grb,
+                       s->gB[0], s->gB[1]);
+    LAUNCH_CHECK("k_init_ghost");
+    HIP_TRY(hipMemcpyAsync(s->bin[1], s->bin[0], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->bout[1], s->bout[0], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->stats, 0, sizeof(double) * 7 * s->bins(), s->stream));
+    HIP_TRY(hipMemsetAsync(s->count, 0, sizeof(unsigned long long) * s->bins(), s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->step = 0;
+    s->initialized = true;
+    s->moments_valid = true;  // all-zero interior moments before the first step (memory.cuh:85-94)
+    s->inlet_reset_pending = (c.y_mode == MCG_Y_JET);
+    s->started = false;
+    s->start_step = 0;
+    s->samples = 0;
+    return MCG_OK;
+}
+
+int one_step(mcg_solver *s) {
+    // update_radial_statistics_start, meanF/meanFieldsRT.cuh:173-189
+    if (!s->started && s->step >= s->c.NSTATS_START_STEP) {
+        s->started = true;
+        s->start_step = s->step;
+        s->samples = 0;
+    }
+    const int64_t every = s->c.NSTATS_SAMPLE > 0 ? s->c.NSTATS_SAMPLE : 1;
+    const bool do_stats = s->started && ((s->step - s->start_step) % every == 0);
+    const DevBufs B = s->bufs(do_stats);
+    if (s->kernel == MCG_KERNEL_TWOPASS) {
+        int rc = launch_moments(s, s->cur);
+        if (rc) return rc;
+        if (do_stats) {
+            const dim3 g(unsigned(s->P.IY), unsigned((s->P.IZ + kRadialZChunk - 1) / kRadialZChunk));
+            hipLaunchKernelGGL(k_radial, g, dim3(256), size_t(8) * s->P.NR * sizeof(double), s->stream, s->P, s->mom,
+                               s->stats, s->count);
+            LAUNCH_CHECK("k_radial");
+        }
+        hipLaunchKernelGGL(k_collide, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, B);
+        LAUNCH_CHECK("k_collide");
+        s->moments_valid = true;
+    } else {
+        DevBufs Bf = B;
+        bool have_moments = false;
+        if (do_stats && !fused_has_stats()) {
+            // statistics from the moments of this step, LDS-privatised histogram
+            int rc = launch_moments(s, s->cur);
+            if (rc) return rc;
+            const dim3 g(unsigned(s->P.IY), unsigned((s->P.IZ + kRadialZChunk - 1) / kRadialZChunk));
+            hipLaunchKernelGGL(k_radial, g, dim3(256), size_t(8) * s->P.NR * sizeof(double), s->stream, s->P, s->mom,
+                               s->stats, s->count);
+            LAUNCH_CHECK("k_radial");
+            Bf.do_stats = 0;
+            have_moments = true;
+        }
+        const int rc = launch_fused(s->P, Bf, s->stream);
+        if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
+        s->moments_valid = have_moments;
+    }
+    if (do_stats) ++s->samples;
+    if (s->inlet_reset_pending) {
+        hipLaunchKernelGGL(k_inlet_ghost_reset, dim3(unsigned((s->c.NX - 2 + 255) / 256), unsigned(s->c.NZ - 2)), dim3(256), 0,
+                           s->stream, s->P, s->grr, s->grb);
+        LAUNCH_CHECK("k_inlet_ghost_reset");
+        s->inlet_reset_pending = false;
+    }
+    s->cur ^= 1;
+    ++s->step;
+    return MCG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mcg_last_error(void) { return g_err.c_str(); }
+const char *mcg_version(void) { return "mcg-mi355x 0.1.0 (gfx950)"; }
+
+int mcg_case_reference_jet(mcg_case *c) {
+    if (!c) return fail(MCG_E_INVALID, "null case");
+    std::memset(c, 0, sizeof(*c));
+    // constants.cuh:8-76
+    c->NX = 128;
+    c->NZ = 128;
+    c->NY = 400;
+    c->NSTEP = 200000;
+    c->NOUTPUT = 2000;
+    c->NSTATS_START_STEP = 6000;
+    c->NSTATS_SAMPLE = 1;
+    c->write_vti_output = 1;
+    c->sponge_cells = 34;
+    c->y_mode = MCG_Y_JET;
+    c->wrap_gradient = 0;
+    c->viscosity_mode = MCG_VISC_SINGLE;
+    c->inlet_profile = MCG_INLET_SHARP;
+    c->jet_radius = 7.0;
+    c->jet_x0 = double(c->NX - 1) / 2.0;
+    c->jet_z0 = double(c->NZ - 1) / 2.0;
+    c->jet_velocity = 0.05;
+    c->bubble_radius = 10.0;
+    c->bubble_x0 = double(c->NX - 1) / 2.0;
+    c->bubble_y0 = double(c->NY - 1) / 2.0;
+    c->bubble_z0 = double(c->NZ - 1) / 2.0;
+    c->rhor0 = 1.0;
+    c->rhob0 = 1.0;
+    c->Re = 5000.0;
+    const double nu = (2.0 * c->jet_radius * c->jet_velocity) / c->Re;
+    c->taur = 0.5 + nu / (1.0 / 3.0);
+    c->taub = 0.5 + nu / (1.0 / 3.0);
+    c->We = 2000.0;
+    c->sigma = (c->rhob0 * c->jet_velocity * c->jet_velocity * 2.0 * c->jet_radius) / c->We;
+    c->beta_recolor = 0.80;
+    c->A_override = 0.0;
+    c->sponge_K = 0.0;
+    c->interface_width = 2.0;
+    c->perturbation_amp = 0.0;
+    return MCG_OK;
+}
+
+size_t mcg_case_size(void) { return sizeof(mcg_case); }
+
+int mcg_case_validate(const mcg_case *c) {
+    if (!c) return fail(MCG_E_INVALID, "null case");
+    if (c->NX < 4 || c->NY < 3 || c->NZ < 4) return fail(MCG_E_INVALID, "grid too small (NX,NZ >= 4, NY >= 3)");
+    if (c->NX > 2048) return fail(MCG_E_INVALID, "NX > 2048 (radial bins NX/2 must fit the LDS histogram)");
+    const size_t px = size_t((c->NX - 2 + 63) / 64 * 64);
+    const size_t plane = px * size_t(c->NY - 2) * size_t(c->NZ - 2);
+    if (plane * 27 >= (size_t(1) << 32)) return fail(MCG_E_INVALID, "grid too large for 32-bit plane offsets");
+    if (c->y_mode != MCG_Y_JET && c->y_mode != MCG_Y_PERIODIC) return fail(MCG_E_INVALID, "unknown y_mode");
+    if (c->viscosity_mode != MCG_VISC_SINGLE && c->viscosity_mode != MCG_VISC_INTERFACE)
+        return fail(MCG_E_INVALID, "unknown viscosity_mode");
+    if (c->inlet_profile != MCG_INLET_SHARP) return fail(MCG_E_UNSUPPORTED, "only the sharp inlet profile is implemented");
+    if (!(c->taub > 0.5) || !(c->taur > 0.5)) return fail(MCG_E_INVALID, "relaxation times must exceed 0.5");
+    if (c->sponge_cells < 2 && c->sponge_K != 0.0) return fail(MCG_E_INVALID, "sponge_cells must be >= 2");
+    if (c->rhor0 <= 0 || c->rhob0 <= 0) return fail(MCG_E_INVALID, "reference densities must be > 0");
+    return MCG_OK;
+}
+
+int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
+    if (!out) return fail(MCG_E_INVALID, "null output pointer");
+    *out = nullptr;
+    int rc = mcg_case_validate(c);
+    if (rc) return rc;
+    if (storage != MCG_STORE_FP32) return fail(MCG_E_UNSUPPORTED, "storage mode not implemented");
+    mcg_solver *s = new (std::nothrow) mcg_solver();
+    if (!s) return fail(MCG_E_NOMEM, "host allocation failed");
+    s->c = *c;
+    s->storage = storage;
+    s->stream = static_cast<hipStream_t>(stream);
+    fill_params(*c, storage, s->P);
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        delete s;
+        return fail(MCG_E_HIP, "hipGetDevice failed (no GPU?)");
+    }
+    auto alloc = [&](void **p, size_t nb) -> bool {
+        if (hipMalloc(p, nb) != hipSuccess) return false;
+        s->bytes += nb;
+        return true;
+    };
+    bool ok = true;
+    for (int b = 0; b < 2 && ok; ++b) {
+        ok = ok && alloc((void **)&s->f[b], s->pop_bytes());
+        ok = ok && alloc((void **)&s->bin[b], s->bnd_bytes());
+        ok = ok && alloc((void **)&s->bout[b], s->bnd_bytes());
+        ok = ok && alloc((void **)&s->gB[b], s->gB_bytes());
+    }
+    ok = ok && alloc((void **)&s->grr, s->N() * sizeof(float));
+    ok = ok && alloc((void **)&s->grb, s->N() * sizeof(float));
+    ok = ok && alloc((void **)&s->mom, size_t(11) * s->P.plane * sizeof(float));
+    ok = ok && alloc((void **)&s->stats, sizeof(double) * 7 * s->bins());
+    ok = ok && alloc((void **)&s->count, sizeof(unsigned long long) * s->bins());
+    s->red_blocks = 1024;
+    ok = ok && alloc((void **)&s->red_partials, sizeof(double) * kRedQ * s->red_blocks);
+    ok = ok && alloc((void **)&s->red_out, sizeof(double) * kRedQ);
+    if (!ok) {
+        mcg_destroy(s);
+        return fail(MCG_E_NOMEM, "device allocation failed");
+    }
+    *out = s;
+    return MCG_OK;
+}
+
+int mcg_destroy(mcg_solver *s) {
+    if (!s) return MCG_OK;
+    for (int b = 0; b < 2; ++b) {
+        for (float *p : {s->f[b], s->bin[b], s->bout[b], s->gB[b]})
+            if (p) (void)hipFree(p);
+    }
+    for (void *p : {(void *)s->grr, (void *)s->grb, (void *)s->mom, (void *)s->stats, (void *)s->count,
+                    (void *)s->red_partials, (void *)s->red_out})
+        if (p) (void)hipFree(p);
+    s->release_staging();
+    delete s;
+    return MCG_OK;
+}
+
+int mcg_set_kernel(mcg_solver *s, int kernel) {
+    if (int rc = check_solver(s)) return rc;
+    if (kernel != MCG_KERNEL_FUSED && kernel != MCG_KERNEL_TWOPASS) return fail(MCG_E_INVALID, "unknown kernel");
+    s->kernel = kernel;
+    return MCG_OK;
+}
+
+int mcg_init_jet(mcg_solver *s) { return do_init(s, kInitJet); }
+int mcg_init_bubble(mcg_solver *s) { return do_init(s, kInitBubble); }
+int mcg_init_bubble_tanh(mcg_solver *s) { return do_init(s, kInitBubbleTanh); }
+
+int mcg_step(mcg_solver *s, int nsteps) {
+    if (int rc = check_ready(s)) return rc;
+    if (nsteps < 0) return fail(MCG_E_INVALID, "nsteps < 0");
+    for (int k = 0; k < nsteps; ++k) {
+        const int rc = one_step(s);
+        if (rc) return rc;
+    }
+    return MCG_OK;
+}
+
+int mcg_synchronize(mcg_solver *s) {
+    if (int rc = check_solver(s)) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MCG_OK;
+}
+
+int mcg_get_step(const mcg_solver *s, int64_t *step) {
+    if (int rc = check_solver(s)) return rc;
+    if (!step) return fail(MCG_E_INVALID, "null pointer");
+    *step = s->step;
+    return MCG_OK;
+}
+
+int mcg_stats_info(const mcg_solver *s, int32_t *started, int64_t *start_step, int64_t *samples) {
+    if (int rc = check_solver(s)) return rc;
+    if (started) *started = s->started ? 1 : 0;
+    if (start_step) *start_step = s->start_step;
+    if (samples) *samples = s->samples;
+    return MCG_OK;
+}
+
+int mcg_stats_reset(mcg_solver *s) {
+    if (int rc = check_ready(s)) return rc;
+    HIP_TRY(hipMemsetAsync(s->stats, 0, sizeof(double) * 7 * s->bins(), s->stream));
+    HIP_TRY(hipMemsetAsync(s->count, 0, sizeof(unsigned long long) * s->bins(), s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->started = false;
+    s->start_step = 0;
+    s->samples = 0;
+    return MCG_OK;
+}
+
+int mcg_copy_radial(mcg_solver *s, double *sum_uy, double *sum_uy2, double *sum_ur, double *sum_ur2, double *sum_uruy,
+                    uint64_t *count, double *sum_phi, double *sum_phi2) {
+    if (int rc = check_ready(s)) return rc;
+    const size_t nb = s->bins();
+    double *dst[7] = {sum_uy, sum_uy2, sum_ur, sum_ur2, sum_uruy, sum_phi, sum_phi2};
+    for (int k = 0; k < 7; ++k)
+        if (dst[k]) HIP_TRY(hipMemcpyAsync(dst[k], s->stats + k * nb, nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    if (count) HIP_TRY(hipMemcpyAsync(count, s->count, nb * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MCG_OK;
+}
+
+int mcg_tke(mcg_solver *s, double *tke_total) {
+    if (int rc = check_ready(s)) return rc;
+    if (!tke_total) return fail(MCG_E_INVALID, "null pointer");
+    if (int rc = ensure_moments(s)) return rc;
+    double r[kRedQ];
+    if (int rc = run_reduce(s, r)) return rc;
+    *tke_total = r[0];
+    return MCG_OK;
+}
+
+int mcg_global_sums(mcg_solver *s, double out[5]) {
+    if (int rc = check_ready(s)) return rc;
+    if (!out) return fail(MCG_E_INVALID, "null pointer");
+    if (int rc = ensure_moments(s)) return rc;
+    double r[kRedQ];
+    if (int rc = run_reduce(s, r)) return rc;
+    for (int k = 0; k < 5; ++k) out[k] = r[1 + k];
+    return MCG_OK;
+}
+
+int mcg_check_health(mcg_solver *s) {
+    if (int rc = check_ready(s)) return rc;
+    if (s->step == 0) return MCG_OK;
+    if (int rc = ensure_moments(s)) return rc;
+    double r[kRedQ];
+    if (int rc = run_reduce(s, r)) return rc;
+    if (r[6] > 0) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "watchdog: %.0f interior nodes with non-finite or non-positive density/velocity at step %lld",
+                      r[6], (long long)s->step);
+        return fail(MCG_E_NUMERIC, buf);
+    }
+    return MCG_OK;
+}
+
+int mcg_copy_fields(mcg_solver *s, float *rhor, float *rhob, float *ux, float *uy, float *uz) {
+    if (int rc = check_ready(s)) return rc;
+    if (int rc = ensure_moments(s)) return rc;
+    const size_t N = s->N();
+    if (int rc = s->ensure_staging(5 * N * sizeof(float))) return rc;
+    hipLaunchKernelGGL(k_export_fields, s->grid_full(), dim3(64, 4, 1), 0, s->stream, s->P, s->mom, 0, s->gB[s->cur], s->grr,
+                       s->grb, s->staging);
+    LAUNCH_CHECK("k_export_fields");
+    float *dst[5] = {rhor, rhob, ux, uy, uz};
+    for (int k = 0; k < 5; ++k)
+        if (dst[k]) HIP_TRY(hipMemcpyAsync(dst[k], s->staging + k * N, N * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MCG_OK;
+}
+
+int mcg_copy_populations(mcg_solver *s, float *fr, float *fb) {
+    if (int rc = check_ready(s)) return rc;
+    const size_t n = size_t(Q) * s->N();
+    if (int rc = s->ensure_staging(2 * n * sizeof(float))) return rc;
+    HIP_TRY(hipMemsetAsync(s->staging, 0, 2 * n * sizeof(float), s->stream));
+    hipLaunchKernelGGL(k_gather_pops, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->f[s->cur], s->bin[s->cur],
+                       s->bout[s->cur], s->staging, s->staging + n);
+    LAUNCH_CHECK("k_gather_pops");
+    if (fr) HIP_TRY(hipMemcpyAsync(fr, s->staging, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    if (fb) HIP_TRY(hipMemcpyAsync(fb, s->staging + n, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MCG_OK;
+}
+
+int mcg_set_populations(mcg_solver *s, const float *fr, const float *fb) {
+    if (int rc = check_ready(s)) return rc;
+    if (!fr || !fb) return fail(MCG_E_INVALID, "null pointer");
+    const size_t n = size_t(Q) * s->N();
+    if (int rc = s->ensure_staging(2 * n * sizeof(float))) return rc;
+    HIP_TRY(hipMemcpyAsync(s->staging, fr, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->staging + n, fb, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_scatter_pops, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->f[s->cur], s->bin[s->cur],
+                       s->bout[s->cur], s->staging, s->staging + n);
+    LAUNCH_CHECK("k_scatter_pops");
+    // entries the boundary never rewrites must agree in both copies
+    HIP_TRY(hipMemcpyAsync(s->bin[s->cur ^ 1], s->bin[s->cur], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->bout[s->cur ^ 1], s->bout[s->cur], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MCG_OK;
+}
+
+// checkpoint image: header, populations (current), boundary buffers (current),
+// outlet ghost state (current), ghost densities, moments, statistics
+struct CkptHeader {
+    char magic[8];
+    int32_t NX, NY, NZ, storage;
+    int64_t step, start_step, samples;
+    int32_t started, inlet_reset_pending, moments_valid, reserved;
+};
+
+static size_t ckpt_bytes(const mcg_solver *s) {
+    return sizeof(CkptHeader) + s->pop_bytes() + 2 * s->bnd_bytes() + s->gB_bytes() + 2 * s->N() * sizeof(float) +
+           size_t(11) * s->P.plane * sizeof(float) + sizeof(double) * 7 * s->bins() + sizeof(uint64_t) * s->bins();
+}
+
+int mcg_checkpoint_size(const mcg_solver *s, uint64_t *nbytes) {
+    if (int rc = check_solver(s)) return rc;
+    if (!nbytes) return fail(MCG_E_INVALID, "null pointer");
+    *nbytes = ckpt_bytes(s);
+    return MCG_OK;
+}
+
+int mcg_checkpoint_save(mcg_solver *s, void *host_buf, uint64_t nbytes) {
+    if (int rc = check_ready(s)) return rc;
+    if (!host_buf || nbytes < ckpt_bytes(s)) return fail(MCG_E_INVALID, "checkpoint buffer too small");
+    char *p = static_cast<char *>(host_buf);
+    CkptHeader h{};
+    std::memcpy(h.magic, "MCGCKPT1", 8);
+    h.NX = s->c.NX; h.NY = s->c.NY; h.NZ = s->c.NZ; h.storage = s->storage;
+    h.step = s->step; h.start_step = s->start_step; h.samples = s->samples;
+    h.started = s->started; h.inlet_reset_pending = s->inlet_reset_pending; h.moments_valid = s->moments_valid;
+    if (int rc = ensure_moments(s)) return rc;
+    h.moments_valid = 1;
+    std::memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    auto d2h = [&](const void *src, size_t nb) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(p, src, nb, hipMemcpyDeviceToHost, s->stream);
+        p += nb;
+        return e;
+    };
+    HIP_TRY(d2h(s->f[s->cur], s->pop_bytes()));
+    HIP_TRY(d2h(s->bin[s->cur], s->bnd_bytes()));
+    HIP_TRY(d2h(s->bout[s->cur], s->bnd_bytes()));
+    HIP_TRY(d2h(s->gB[s->cur], s->gB_bytes()));
+    HIP_TRY(d2h(s->grr, s->N() * sizeof(float)));
+    HIP_TRY(d2h(s->grb, s->N() * sizeof(float)));
+    HIP_TRY(d2h(s->mom, size_t(11) * s->P.plane * sizeof(float)));
+    HIP_TRY(d2h(s->stats, sizeof(double) * 7 * s->bins()));
+    HIP_TRY(d2h(s->count, sizeof(uint64_t) * s->bins()));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MCG_OK;
+}
+
+int mcg_checkpoint_load(mcg_solver *s, const void *host_buf, uint64_t nbytes) {
+    if (int rc = check_solver(s)) return rc;
+    if (!host_buf || nbytes < ckpt_bytes(s)) return fail(MCG_E_INVALID, "checkpoint buffer too small");
+    const char *p = static_cast<const char *>(host_buf);
+    CkptHeader h;
+    std::memcpy(&h, p, sizeof(h));
+    if (std::memcmp(h.magic, "MCGCKPT1", 8) != 0) return fail(MCG_E_INVALID, "not a checkpoint image");
+    if (h.NX != s->c.NX || h.NY != s->c.NY || h.NZ != s->c.NZ || h.storage != s->storage)
+        return fail(MCG_E_INVALID, "checkpoint was taken for another grid or storage mode");
+    p += sizeof(h);
+    s->cur = 0;
+    auto h2d = [&](void *dst, size_t nb) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(dst, p, nb, hipMemcpyHostToDevice, s->stream);
+        p += nb;
+        return e;
+    };
+    HIP_TRY(h2d(s->f[0], s->pop_bytes()));
+    HIP_TRY(h2d(s->bin[0], s->bnd_bytes()));
+    HIP_TRY(h2d(s->bout[0], s->bnd_bytes()));
+    HIP_TRY(h2d(s->gB[0], s->gB_bytes()));
+    HIP_TRY(h2d(s->grr, s->N() * sizeof(float)));
+    HIP_TRY(h2d(s->grb, s->N() * sizeof(float)));
+    HIP_TRY(h2d(s->mom, size_t(11) * s->P.plane * sizeof(float)));
+    HIP_TRY(h2d(s->stats, sizeof(double) * 7 * s->bins()));
+    HIP_TRY(h2d(s->count, sizeof(uint64_t) * s->bins()));
+    HIP_TRY(hipMemcpyAsync(s->bin[1], s->bin[0], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->bout[1], s->bout[0], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->gB[1], s->gB[0], s->gB_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->step = h.step;
+    s->start_step = h.start_step;
+    s->samples = h.samples;
+    s->started = h.started != 0;
+    s->inlet_reset_pending = h.inlet_reset_pending != 0;
+    s->moments_valid = true;
+    s->initialized = true;
+    return MCG_OK;
+}
+
+int mcg_time_steps(mcg_solver *s, int nsteps, double *ms) {
+    if (int rc = check_ready(s)) return rc;
+    if (!ms) return fail(MCG_E_INVALID, "null pointer");
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, s->stream));
+    int rc = mcg_step(s, nsteps);
+    if (rc) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return rc;
+    }
+    HIP_TRY(hipEventRecord(b, s->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = double(t);
+    return MCG_OK;
+}
+
+int mcg_device_bytes(const mcg_solver *s, uint64_t *nbytes) {
+    if (int rc = check_solver(s)) return rc;
+    if (!nbytes) return fail(MCG_E_INVALID, "null pointer");
+    *nbytes = s->bytes;
+    return MCG_OK;
+}
+
+}  // extern "C"

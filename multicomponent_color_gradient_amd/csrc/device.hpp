@@ -1,0 +1,281 @@
+// device.hpp -- device-side state description, HBM layout and the per-node
+// streaming (pull) access used by every step kernel.
+//
+// HBM layout (FP32 storage)
+//   populations  : two buffers (read "src", write "dst"), each 54 planes
+//                  [colour*27 + i][plane], plane = PX * (NY-2) * (NZ-2),
+//                  interior nodes only, x fastest with the row padded to PX
+//                  (a multiple of 64) so that every wavefront row is aligned.
+//                  A buffer holds the post-collision populations of the last
+//                  step stored at their SOURCE node: the step gathers
+//                  f_i(x) = f*_i(x - c_i) (pull), which reproduces the
+//                  reference's in-place push (lbm.cuh:230-244) exactly.
+//   inlet/outlet : [slot k][colour][NZ][NX], k indexes the 9 directions with
+//                  c_y=+1 (inlet) / c_y=-1 (outlet).  Entry (x,z) holds the
+//                  population that the ghost node (x,0,z) / (x,NY-1,z) sends
+//                  into the first/last fluid plane, double buffered.  Entries
+//                  the reference never writes keep their initial values
+//                  (quirks Q1/Q2: jet/boundary_conditions.cuh:102-105 and the
+//                  unwrapped targets of :137,:286).
+//   outlet ghost : [5][NZ][NX] rho_r, rho_b, u_x, u_y, u_z of plane y=NY-1,
+//                  double buffered (the colour gradient of step n reads the
+//                  values of step n-1, jet/boundary_conditions.cuh:256-260).
+//   ghost rho    : full-grid rho_r, rho_b holding the values of ghost nodes
+//                  (x/z frames and the y=0 plane), read by the colour
+//                  gradient without wrap (forces.cuh:31-33).
+//   moments      : 11 interior planes (rho_r, rho_b, u, Pi), written by the
+//                  two-pass path and by the on-demand moments pass.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/mcg.h"
+#include "physics.hpp"
+
+namespace mcg {
+
+struct DevParams {
+    int NX, NY, NZ;
+    int PX;           // padded interior row pitch (elements)
+    int IY, IZ;       // NY-2, NZ-2
+    uint32_t plane;   // PX*IY*IZ
+    int y_mode, wrap_gradient, visc_mode;
+    int sponge_ystart, sponge_yend;
+    float nu, sponge_K;
+    float taur, taub;
+    float omega_b;    // 1/taub, the inlet's omegab (constants.cuh:67)
+    float A_fixed;    // > 0: A used as given
+    float sig4;       // sigma/(4 cs^4), constants.cuh:76
+    float beta;
+    float jet_x0, jet_z0, jet_R, jet_U;
+    float rhor0, rhob0;
+    int NR;           // radial bins, NX/2 (constants.cuh:20)
+    int storage;      // MCG_STORE_*
+};
+
+struct DevBufs {
+    const float *f_src;
+    float *f_dst;
+    const float *in_src;
+    float *in_dst;
+    const float *out_src;
+    float *out_dst;
+    const float *gB_src;   // outlet ghost state read this step
+    float *gB_dst;         // outlet ghost state written this step
+    const float *ghost_rr;
+    const float *ghost_rb;
+    float *mom;            // 11 planes
+    double *stats;         // 7 planes of NY*NR (uy, uy2, ur, ur2, uruy, phi, phi2)
+    unsigned long long *count;  // NY*NR
+    int do_stats;
+};
+
+__device__ __forceinline__ uint32_t pidx(const DevParams &P, int x, int y, int z) {
+    return uint32_t(x - 1) + uint32_t(P.PX) * (uint32_t(y - 1) + uint32_t(P.IY) * uint32_t(z - 1));
+}
+__device__ __forceinline__ size_t gidx(const DevParams &P, int x, int y, int z) {
+    return size_t(x) + size_t(P.NX) * (size_t(y) + size_t(P.NY) * size_t(z));
+}
+// periodic wrap over the ghost layers, utilities/bounds.cuh:23-60
+__device__ __forceinline__ int wrapN(int v, int n) { return v == 0 ? n - 2 : (v == n - 1 ? 1 : v); }
+
+__device__ __forceinline__ bool is_jet(const DevParams &P, int x, int z) {  // utilities/mathUtilities.cuh:14-21
+    const float dx = float(x) - P.jet_x0;
+    const float dz = float(z) - P.jet_z0;
+    return dx * dx + dz * dz <= P.jet_R * P.jet_R;
+}
+
+// relaxation rate of the sponge layer, utilities/mathUtilities.cuh:49-90
+__device__ __forceinline__ float omega_sponge(const DevParams &P, int y) {
+    const float inv_width = 1.0f / float(P.sponge_yend - P.sponge_ystart);
+    const float s = clamp01((float(y) - float(P.sponge_ystart)) * inv_width);
+    const float nus = P.nu * (1.0f + P.sponge_K * (s * s * s));
+    const float tau = 0.5f + nus * inv_cs2;
+    return 1.0f / tau;
+}
+// interface relaxation time, forces.cuh:77-85
+__device__ __forceinline__ float tau_interface(const DevParams &P, float rr, float rb) {
+    const float p = psi(rr, rb);
+    return ((1.0f + p) / 2.0f) * P.taur + ((1.0f - p) / 2.0f) * P.taub;
+}
+__device__ __forceinline__ float omega_node(const DevParams &P, int y, float rr, float rb) {
+    if (P.visc_mode == MCG_VISC_INTERFACE) return 1.0f / tau_interface(P, rr, rb);
+    return omega_sponge(P, y);
+}
+// A = sigma/(4 cs^4 tau), forces.cuh:87-90,102-103
+__device__ __forceinline__ float A_node(const DevParams &P, float rr, float rb) {
+    if (P.A_fixed > 0.0f) return P.A_fixed;
+    const float tau = P.visc_mode == MCG_VISC_INTERFACE ? tau_interface(P, rr, rb) : P.taub;
+    return P.sig4 / tau;
+}
+
+// Location of the population that streams into interior node (x,y,z) along
+// direction I: (base pointer, element offset of colour r, stride to colour b).
+struct SrcRef {
+    const float *base;
+    size_t off;
+    size_t cstride;
+};
+
+template <int I>
+__device__ __forceinline__ SrcRef source_of(const DevParams &P, const float *f, const float *bin, const float *bout,
+                                            int x, int y, int z) {
+    int xs = x - kCX[I];
+    int ys = y - kCY[I];
+    int zs = z - kCZ[I];
+    if constexpr (kCY[I] != 0) {
+        if (P.y_mode == MCG_Y_JET) {
+            const bool ghost = kCY[I] > 0 ? (ys == 0) : (ys == P.NY - 1);
+            if (ghost) {
+                // ghost-plane sources are not wrapped (jet/boundary_conditions.cuh:137,286)
+                const size_t nxz = size_t(P.NX) * size_t(P.NZ);
+                const size_t off = size_t(slot_of(I)) * 2 * nxz + size_t(zs) * P.NX + size_t(xs);
+                return SrcRef{kCY[I] > 0 ? bin : bout, off, nxz};
+            }
+        } else {
+            ys = wrapN(ys, P.NY);
+        }
+    }
+    xs = wrapN(xs, P.NX);
+    zs = wrapN(zs, P.NZ);
+    const size_t off = size_t(I) * P.plane + pidx(P, xs, ys, zs);
+    return SrcRef{f, off, size_t(Q) * P.plane};
+}
+
+template <int I>
+__device__ __forceinline__ void gather(const DevParams &P, const float *__restrict__ f, const float *__restrict__ bin,
+                                       const float *__restrict__ bout, int x, int y, int z, float &fr, float &fb) {
+    const SrcRef s = source_of<I>(P, f, bin, bout, x, y, z);
+    fr = s.base[s.off];
+    fb = s.base[s.off + s.cstride];
+}
+
+// moments of interior node (x,y,z) from the gathered populations (lbm.cuh:82-165)
+__device__ __forceinline__ Moments gather_moments(const DevParams &P, const float *__restrict__ f,
+                                                  const float *__restrict__ bin, const float *__restrict__ bout,
+                                                  int x, int y, int z) {
+    MomentAcc acc;
+    unroll<0, Q>([&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        float a, b;
+        gather<I>(P, f, bin, bout, x, y, z, a, b);
+        acc.add<I>(a, b);
+    });
+    return acc.finish();
+}
+
+__device__ __forceinline__ bool interior(const DevParams &P, int x, int y, int z) {
+    return x >= 1 && x <= P.NX - 2 && y >= 1 && y <= P.NY - 2 && z >= 1 && z <= P.NZ - 2;
+}
+
+// psi at a ghost node, as the colour gradient reads it (forces.cuh:31-33):
+// the outlet plane's inner nodes hold the convective state of the previous
+// step, every other ghost node its ghost density.
+__device__ __forceinline__ float ghost_psi(const DevParams &P, const DevBufs &B, int x, int y, int z) {
+    if (P.y_mode == MCG_Y_JET && y == P.NY - 1 && x >= 1 && x <= P.NX - 2 && z >= 1 && z <= P.NZ - 2) {
+        const size_t nxz = size_t(P.NX) * P.NZ;
+        const size_t o = size_t(z) * P.NX + x;
+        return psi(B.gB_src[o], B.gB_src[nxz + o]);
+    }
+    const size_t g = gidx(P, x, y, z);
+    return psi(B.ghost_rr[g], B.ghost_rb[g]);
+}
+
+// neighbour coordinates for the colour gradient (wrapped when wrap_gradient)
+__device__ __forceinline__ void grad_neighbor(const DevParams &P, int x, int y, int z, int &xn, int &yn, int &zn) {
+    xn = x; yn = y; zn = z;
+    if (P.wrap_gradient) {
+        xn = wrapN(x, P.NX);
+        zn = wrapN(z, P.NZ);
+        if (P.y_mode == MCG_Y_PERIODIC) yn = wrapN(y, P.NY);
+    }
+}
+
+// Per-node collision set-up, lbm.cuh:193-250 and forces.cuh:92-105.
+template <class PhiFn>
+__device__ __forceinline__ NodeColl node_setup(const DevParams &P, const Moments &m, int y, PhiFn &&phi) {
+    NodeColl n;
+    n.m = m;
+    n.rT = m.rr + m.rb;
+    const float invrT = 1.0f / n.rT;
+    n.aR = m.rr * invrT;
+    n.aB = m.rb * invrT;
+    n.oms = 1.0f - omega_node(P, y, m.rr, m.rb);
+    const float I = 4.0f * m.rr * m.rb * invrT * invrT;
+    n.bulk = I <= 1.0e-4f;
+    n.Fx = n.Fy = n.Fz = n.absF = n.invF = n.A = n.coeff = 0.0f;
+    if (!n.bulk) {
+        color_gradient(phi, n.Fx, n.Fy, n.Fz);
+        n.A = A_node(P, m.rr, m.rb);
+        n.absF = sqrtf(n.Fx * n.Fx + n.Fy * n.Fy + n.Fz * n.Fz);
+        n.invF = 1.0f / n.absF;
+        n.coeff = P.beta * ((m.rr * m.rb) / (n.rT * n.rT));
+    }
+    return n;
+}
+
+// Inlet populations sent by ghost node (x,0,z) of a jet column, computed from
+// the moments m of node (x,1,z): jet/boundary_conditions.cuh:97-148.
+__device__ __forceinline__ void inlet_emit(const DevParams &P, float *__restrict__ in_dst, const Moments &m, int x, int z) {
+    const float rrB = (1.0f - 1.0f) * P.rhor0;  // jet node: is_jet = 1
+    const float rbB = 1.0f * P.rhob0;
+    const float rT = rrB + rbB;
+    const float aR = rrB * (1.0f / rT);
+    const float aB = rbB * (1.0f / rT);
+    float om = P.omega_b;
+    if (P.visc_mode == MCG_VISC_INTERFACE) om = 1.0f / tau_interface(P, rrB, rbB);
+    const float oms = 1.0f - om;
+    const size_t nxz = size_t(P.NX) * P.NZ;
+    const size_t o = size_t(z) * P.NX + x;
+    unroll<0, 9>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr int I = kUp[k];
+        const float g = boundary_pop<I>(rT, 0.0f, P.jet_U, 0.0f, oms, m);
+        in_dst[(2 * k) * nxz + o] = aR * g;
+        in_dst[(2 * k + 1) * nxz + o] = aB * g;
+    });
+}
+
+// Convective outlet for column (x,z) from the moments m of node (x,NY-2,z):
+// jet/boundary_conditions.cuh:163-233.
+__device__ __forceinline__ void outlet_emit(const DevParams &P, const float *__restrict__ gB_src, float *__restrict__ gB_dst,
+                                            float *__restrict__ out_dst, const Moments &m, int x, int z) {
+    const size_t nxz = size_t(P.NX) * P.NZ;
+    const size_t o = size_t(z) * P.NX + x;
+    const float uc = clamp01(m.uy);
+    const float rrB = gB_src[o] - uc * (gB_src[o] - m.rr);
+    const float rbB = gB_src[nxz + o] - uc * (gB_src[nxz + o] - m.rb);
+    const float uxB = gB_src[2 * nxz + o] - uc * (gB_src[2 * nxz + o] - m.ux);
+    const float uyB = gB_src[3 * nxz + o] - uc * (gB_src[3 * nxz + o] - m.uy);
+    const float uzB = gB_src[4 * nxz + o] - uc * (gB_src[4 * nxz + o] - m.uz);
+    gB_dst[o] = rrB;
+    gB_dst[nxz + o] = rbB;
+    gB_dst[2 * nxz + o] = uxB;
+    gB_dst[3 * nxz + o] = uyB;
+    gB_dst[4 * nxz + o] = uzB;
+    const float rhoT = rrB + rbB;
+    const float inv = 1.0f / rhoT;
+    const float aR = rrB * inv;
+    const float aB = rbB * inv;
+    const float oms = 1.0f - omega_node(P, P.NY - 2, rrB, rbB);
+    unroll<0, 9>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr int I = kDown[k];
+        const float g = boundary_pop<I>(rhoT, uxB, uyB, uzB, oms, m);
+        out_dst[(2 * k) * nxz + o] = aR * g;
+        out_dst[(2 * k + 1) * nxz + o] = aB * g;
+    });
+}
+
+// Radial bin of interior column (x,z), kernels.cu:207-215; -1 when outside.
+__device__ __forceinline__ int radial_bin(const DevParams &P, int x, int z, float &dx, float &dz, float &r) {
+    dx = float(x) - P.jet_x0;
+    dz = float(z) - P.jet_z0;
+    r = sqrtf(dx * dx + dz * dz);
+    const uint32_t rb = uint32_t(r);
+    return rb >= uint32_t(P.NR) ? -1 : int(rb);
+}
+
+}  // namespace mcg

@@ -1,0 +1,175 @@
+"""Python binding of the C ABI: ``Solver`` owns one ``mcg_solver`` handle.
+
+Arrays returned to Python use the reference's grid layout, reshaped to
+``(NZ, NY, NX)`` for cell fields (cell id = x + NX*(y + NY*z),
+utilities/indexing.cuh:10-13) and ``(27, NZ, NY, NX)`` for populations
+(population id = i*Ncells + id, utilities/indexing.cuh:15-19).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import McgCase, as_ptr, check, load_library
+
+
+class Solver:
+    """One solver on the current HIP device.
+
+    ``storage`` is ``"fp32"``, ``"fp16"`` or ``"bf16"``; ``kernel`` is
+    ``"fused"`` (default) or ``"twopass"``.  ``stream`` is a raw hipStream_t
+    (int) or None for the null stream.
+    """
+
+    def __init__(self, case: McgCase, storage: str = "fp32", kernel: str = "fused", stream: int | None = None):
+        self.lib = load_library()
+        self.case = case.copy()
+        h = ctypes.c_void_p()
+        check(self.lib.mcg_create(ctypes.byref(self.case), _abi.STORAGE[storage], ctypes.c_void_p(stream or 0),
+                                  ctypes.byref(h)))
+        self._h = h
+        self.storage = storage
+        self.set_kernel(kernel)
+
+    # ---- lifetime
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.mcg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.case.NZ, self.case.NY, self.case.NX)
+
+    @property
+    def ncells(self) -> int:
+        return self.case.NX * self.case.NY * self.case.NZ
+
+    def set_kernel(self, kernel: str) -> None:
+        check(self.lib.mcg_set_kernel(self._h, _abi.KERNELS[kernel]))
+        self.kernel = kernel
+
+    # ---- case API (jet/jetIn.cuh, bubble/bubbleIn.cuh)
+    def init_jet(self) -> None:
+        check(self.lib.mcg_init_jet(self._h))
+
+    def init_bubble(self) -> None:
+        check(self.lib.mcg_init_bubble(self._h))
+
+    def init_bubble_tanh(self) -> None:
+        check(self.lib.mcg_init_bubble_tanh(self._h))
+
+    # ---- hot path
+    def step(self, n: int = 1) -> None:
+        check(self.lib.mcg_step(self._h, int(n)))
+
+    def synchronize(self) -> None:
+        check(self.lib.mcg_synchronize(self._h))
+
+    @property
+    def step_count(self) -> int:
+        v = ctypes.c_int64()
+        check(self.lib.mcg_get_step(self._h, ctypes.byref(v)))
+        return v.value
+
+    def time_steps(self, n: int) -> float:
+        ms = ctypes.c_double()
+        check(self.lib.mcg_time_steps(self._h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    # ---- statistics (meanF/*)
+    def stats_info(self) -> dict:
+        st, ss, ns = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+        check(self.lib.mcg_stats_info(self._h, ctypes.byref(st), ctypes.byref(ss), ctypes.byref(ns)))
+        return {"started": bool(st.value), "start_step": ss.value, "samples": ns.value}
+
+    def stats_reset(self) -> None:
+        check(self.lib.mcg_stats_reset(self._h))
+
+    def radial(self) -> dict:
+        nb = (self.case.NY, self.case.NX // 2)
+        names = ["sum_uy", "sum_uy2", "sum_ur", "sum_ur2", "sum_uruy", "sum_phi", "sum_phi2"]
+        out = {n: np.zeros(nb, np.float64) for n in names}
+        cnt = np.zeros(nb, np.uint64)
+        d = ctypes.c_double
+        check(self.lib.mcg_copy_radial(self._h, as_ptr(out["sum_uy"], d), as_ptr(out["sum_uy2"], d),
+                                       as_ptr(out["sum_ur"], d), as_ptr(out["sum_ur2"], d),
+                                       as_ptr(out["sum_uruy"], d), as_ptr(cnt, ctypes.c_uint64),
+                                       as_ptr(out["sum_phi"], d), as_ptr(out["sum_phi2"], d)))
+        out["count"] = cnt
+        return out
+
+    def tke(self) -> float:
+        v = ctypes.c_double()
+        check(self.lib.mcg_tke(self._h, ctypes.byref(v)))
+        return v.value
+
+    def global_sums(self) -> np.ndarray:
+        out = np.zeros(5, np.float64)
+        check(self.lib.mcg_global_sums(self._h, as_ptr(out, ctypes.c_double)))
+        return out
+
+    def check_health(self) -> None:
+        check(self.lib.mcg_check_health(self._h))
+
+    # ---- state exchange
+    def fields(self) -> dict:
+        arrs = [np.empty(self.shape, np.float32) for _ in range(5)]
+        f = ctypes.c_float
+        check(self.lib.mcg_copy_fields(self._h, *[as_ptr(a, f) for a in arrs]))
+        return dict(zip(["rhor", "rhob", "ux", "uy", "uz"], arrs))
+
+    def populations(self):
+        fr = np.empty((27,) + self.shape, np.float32)
+        fb = np.empty((27,) + self.shape, np.float32)
+        f = ctypes.c_float
+        check(self.lib.mcg_copy_populations(self._h, as_ptr(fr, f), as_ptr(fb, f)))
+        return fr, fb
+
+    def set_populations(self, fr: np.ndarray, fb: np.ndarray) -> None:
+        fr = np.ascontiguousarray(fr, dtype=np.float32).reshape((27,) + self.shape)
+        fb = np.ascontiguousarray(fb, dtype=np.float32).reshape((27,) + self.shape)
+        f = ctypes.c_float
+        check(self.lib.mcg_set_populations(self._h, as_ptr(fr, f), as_ptr(fb, f)))
+
+    # ---- checkpoint / restart
+    def checkpoint(self) -> np.ndarray:
+        n = ctypes.c_uint64()
+        check(self.lib.mcg_checkpoint_size(self._h, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        check(self.lib.mcg_checkpoint_save(self._h, buf.ctypes.data_as(ctypes.c_void_p), n.value))
+        return buf
+
+    def restore(self, buf: np.ndarray) -> None:
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        check(self.lib.mcg_checkpoint_load(self._h, buf.ctypes.data_as(ctypes.c_void_p), buf.size))
+
+    def device_bytes(self) -> int:
+        n = ctypes.c_uint64()
+        check(self.lib.mcg_device_bytes(self._h, ctypes.byref(n)))
+        return n.value
+
+
+# The reference's case API names (jet/jetIn.cuh, bubble/bubbleIn.cuh).
+def jetIn(solver: Solver) -> None:
+    """Jet initial state: jetDensity + Injet (kernels.cu:29-56, main.cu:39-40)."""
+    solver.init_jet()
+
+
+def bubbleIn(solver: Solver) -> None:
+    """Sharp bubble initial state: bubble (kernels.cu:13-27)."""
+    solver.init_bubble()

@@ -1,0 +1,240 @@
+"""GPU parity: HIP kernels (through the C ABI) against the FP64 CPU oracle.
+
+The GPU computes in FP32 like the reference.  Tolerances against the FP64
+oracle are set from the spread between the oracle's own FP32 and FP64
+instantiations on the same cases (|d rho| ~1.5e-6, |d u| ~3e-7 after 20 jet
+steps, radial sums ~2.5e-5 of the largest bin, TKE ~2e-5 relative):
+  <= 20 steps : populations max|d| <= 2e-6 max|f|; |d rho| <= 5e-6;
+                |d u| <= 1e-6; |d psi| <= 1e-5
+  100 steps   : ten times looser
+  per-colour mass: relative <= 1e-6
+  radial statistics: counts exact; per bin |d sum_u*| <= 1e-6 count,
+                     |d sum_phi*| <= 1e-5 count
+  TKE: relative <= 1e-4
+Kernel variants (fused / two-pass) must agree bit for bit.
+"""
+import numpy as np
+import pytest
+
+from conftest import INNER, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rhor", "rhob", "ux", "uy", "uz")
+
+
+def run_pair(mcg, case, steps, init="jet", kernel="fused", precision=64):
+    from oracle.oracle import Oracle
+
+    s = mcg.Solver(case, kernel=kernel)
+    o = Oracle(case, precision)
+    getattr(s, "init_" + init)()
+    getattr(o, "init_" + init)()
+    s.step(steps)
+    o.step(steps)
+    return s, o
+
+
+def assert_fields(s, o, scale=1.0, label=""):
+    """Absolute bounds: |d rho| <= 5e-6 scale, |d u| <= 1e-6 scale, |d psi| <= 1e-5 scale,
+    over the full grid (interior moments and the exported ghost values)."""
+    g, r = s.fields(), o.fields()
+    for k in FIELDS:
+        tol = (5e-6 if k.startswith("rho") else 1e-6) * scale
+        e = float(np.max(np.abs(g[k].astype(np.float64) - r[k])))
+        assert e <= tol, f"{label} {k}: max |d| {e:.3e} > {tol:.1e}"
+    pg = (g["rhor"] - g["rhob"]).astype(np.float64) / (g["rhor"] + g["rhob"]).astype(np.float64)
+    pr = (r["rhor"] - r["rhob"]) / (r["rhor"] + r["rhob"])
+    e = float(np.max(np.abs(pg - pr)))
+    assert e <= 1e-5 * scale, f"{label} psi: max |d| {e:.3e}"
+
+
+def assert_pops(s, o, tol, label=""):
+    gr, gb = s.populations()
+    rr, rb = o.populations()
+    sl = (slice(None),) + INNER
+    er = rel_err(gr[sl], rr[sl])
+    eb = rel_err(gb[sl], rb[sl])
+    assert er <= tol and eb <= tol, f"{label} populations rel err r={er:.3e} b={eb:.3e} > {tol:.1e}"
+
+
+@pytest.mark.parametrize("steps,scale", [(1, 1.0), (10, 1.0), (100, 10.0)])
+def test_jet_matches_oracle(mcg, steps, scale):
+    case = mcg.jet_case(24, 20, 24, jet_radius=4.0, jet_velocity=0.05)
+    s, o = run_pair(mcg, case, steps)
+    assert_fields(s, o, scale, f"jet {steps} steps")
+    assert_pops(s, o, 2e-6 * scale, f"jet {steps} steps")
+
+
+@pytest.mark.parametrize("steps", [10, 50])
+def test_jet_error_is_fp32_rounding(mcg, steps):
+    """The GPU's distance to the FP64 oracle is no larger than 3x the distance
+    of the oracle's own FP32 instantiation to FP64 (per field, max norm)."""
+    from oracle.oracle import Oracle
+
+    case = mcg.jet_case(24, 20, 24, jet_radius=4.0, jet_velocity=0.05, Re=500.0)
+    s, o64 = run_pair(mcg, case, steps)
+    o32 = Oracle(case, 32)
+    o32.init_jet()
+    o32.step(steps)
+    g, r, q = s.fields(), o64.fields(), o32.fields()
+    for k in FIELDS:
+        eg = float(np.max(np.abs(g[k][INNER] - r[k][INNER])))
+        eq = float(np.max(np.abs(q[k][INNER] - r[k][INNER])))
+        assert eg <= 3 * eq + 1e-9, f"{k}: gpu {eg:.3e} vs fp32 restatement {eq:.3e}"
+
+
+def test_jet_odd_grid_matches_oracle(mcg):
+    # odd NX: integer jet centre, rows not a multiple of 64, radial bins hit r=0
+    case = mcg.jet_case(37, 23, 29, jet_radius=5.0, jet_velocity=0.05, Re=500.0)
+    s, o = run_pair(mcg, case, 20)
+    assert_fields(s, o, 1.0, "odd jet")
+    assert_pops(s, o, 2e-6, "odd jet")
+
+
+def test_sharp_bubble_with_jet_boundaries(mcg):
+    # bubbleIn + the reference's jet boundaries (main.cu:39-41 with launch_bubble)
+    case = mcg.jet_case(26, 30, 22, jet_radius=3.0, Re=300.0)
+    case.bubble_radius = 6.0
+    s, o = run_pair(mcg, case, 10, init="bubble")
+    assert_fields(s, o, 1.0, "bubble")
+    assert_pops(s, o, 2e-6, "bubble")
+
+
+def test_periodic_tanh_bubble(mcg):
+    case = mcg.periodic_bubble_case(20, radius=5.0, tau=0.8, A=1e-3, beta=0.7)
+    s, o = run_pair(mcg, case, 20, init="bubble_tanh")
+    assert_fields(s, o, 1.0, "periodic bubble")
+    assert_pops(s, o, 2e-6, "periodic bubble")
+    m0 = o.global_sums()
+    mg = s.global_sums()
+    assert abs(mg[0] - m0[0]) <= 1e-6 * m0[0] and abs(mg[1] - m0[1]) <= 1e-6 * m0[1]
+
+
+def test_interface_viscosity_mode(mcg):
+    case = mcg.periodic_bubble_case(18, radius=4.0, tau=0.8, A=1e-3, beta=0.7)
+    case.viscosity_mode = 1
+    case.taur, case.taub = 0.55, 1.0
+    s, o = run_pair(mcg, case, 10, init="bubble_tanh")
+    assert_fields(s, o, 1.0, "interface tau")
+
+
+def random_state(shape, seed):
+    """Two-phase populations near equilibrium with interfaces everywhere."""
+    rng = np.random.default_rng(seed)
+    from oracle.oracle import lattice
+
+    L = lattice()
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    frac = 0.5 * (1 + np.tanh(2.0 * np.sin(0.7 * x + 0.3 * y) * np.cos(0.5 * z + 0.2 * y)))
+    w = L["w"][:, None, None, None]
+    fr = w * (1.0 - frac)[None] * (1 + 0.05 * rng.standard_normal((27,) + shape))
+    fb = w * frac[None] * (1 + 0.05 * rng.standard_normal((27,) + shape))
+    return fr.astype(np.float32), fb.astype(np.float32)
+
+
+@pytest.mark.parametrize("ymode", [0, 1])
+def test_random_state_one_step(mcg, ymode):
+    from oracle.oracle import Oracle
+
+    case = mcg.jet_case(22, 18, 20, jet_radius=4.0, Re=300.0)
+    case.A_override = 1e-3
+    case.beta_recolor = 0.7
+    if ymode == 1:
+        case.y_mode, case.wrap_gradient = 1, 1
+    s = mcg.Solver(case)
+    o = Oracle(case)
+    s.init_jet()
+    o.init_jet()
+    fr, fb = random_state(s.shape, 7)
+    s.set_populations(fr, fb)
+    o.set_populations(fr.astype(np.float64), fb.astype(np.float64))
+    gr, gb = s.populations()
+    sl = (slice(None),) + INNER
+    assert np.array_equal(gr[sl], fr[sl]) and np.array_equal(gb[sl], fb[sl])
+    for n in (1, 3):
+        s.step(n)
+        o.step(n)
+        assert_pops(s, o, 5e-6, f"random ymode={ymode}")
+        assert_fields(s, o, 1.0, f"random ymode={ymode}")
+
+
+def test_radial_statistics(mcg):
+    case = mcg.jet_case(30, 22, 30, jet_radius=5.0, Re=500.0)
+    case.NSTATS_START_STEP = 3
+    case.NSTATS_SAMPLE = 2
+    s, o = run_pair(mcg, case, 12)
+    assert s.stats_info() == o.stats_info()
+    g, r = s.radial(), o.radial()
+    assert np.array_equal(g["count"], r["count"])
+    cnt = r["count"].astype(np.float64)
+    for k in ("sum_uy", "sum_uy2", "sum_ur", "sum_ur2", "sum_uruy", "sum_phi", "sum_phi2"):
+        tol = (1e-5 if "phi" in k else 1e-6) * cnt + 1e-12
+        bad = np.abs(g[k] - r[k]) > tol
+        assert not bad.any(), f"{k}: {int(bad.sum())} bins out of tolerance, worst {np.max(np.abs(g[k]-r[k])/tol):.2f}x"
+
+
+@pytest.mark.parametrize("kernel", ["fused", "twopass"])
+def test_radial_statistics_kernels(mcg, kernel):
+    case = mcg.jet_case(26, 20, 26, jet_radius=4.0, Re=500.0)
+    case.NSTATS_START_STEP = 0
+    s, o = run_pair(mcg, case, 6, kernel=kernel)
+    g, r = s.radial(), o.radial()
+    assert np.array_equal(g["count"], r["count"])
+    assert np.all(np.abs(g["sum_uy"] - r["sum_uy"]) <= 1e-6 * r["count"] + 1e-12)
+
+
+def test_tke_and_global_sums(mcg):
+    case = mcg.jet_case(24, 20, 24, jet_radius=4.0, Re=500.0)
+    s, o = run_pair(mcg, case, 15)
+    assert abs(s.tke() - o.tke()) <= 1e-4 * abs(o.tke())
+    gs, os_ = s.global_sums(), o.global_sums()
+    assert np.allclose(gs[:2], os_[:2], rtol=1e-6)
+    # momentum sums cancel; bound the error per interior node (|d u| <= 1e-6)
+    assert np.all(np.abs(gs[2:] - os_[2:]) <= 1e-6 * mcg.interior_nodes(case))
+    s.check_health()
+
+
+def test_fused_equals_twopass_bitwise(mcg):
+    case = mcg.jet_case(40, 36, 34, jet_radius=6.0, Re=500.0)
+    case.NSTATS_START_STEP = 5
+    a = mcg.Solver(case, kernel="fused")
+    b = mcg.Solver(case, kernel="twopass")
+    a.init_jet()
+    b.init_jet()
+    a.step(25)
+    b.step(25)
+    fa, fb_ = a.populations(), b.populations()
+    assert np.array_equal(fa[0], fb_[0]) and np.array_equal(fa[1], fb_[1])
+    ra, rb = a.radial(), b.radial()
+    assert np.array_equal(ra["count"], rb["count"])
+    assert np.allclose(ra["sum_uy"], rb["sum_uy"], rtol=1e-12, atol=1e-15)
+    ga, gb = a.fields(), b.fields()
+    for k in FIELDS:
+        assert np.array_equal(ga[k], gb[k]), k
+
+
+def test_checkpoint_roundtrip_bitwise(mcg):
+    case = mcg.jet_case(24, 22, 24, jet_radius=4.0, Re=500.0)
+    case.NSTATS_START_STEP = 2
+    a = mcg.Solver(case)
+    a.init_jet()
+    a.step(7)
+    img = a.checkpoint()
+    a.step(5)
+    ref = a.populations()
+    b = mcg.Solver(case)
+    b.restore(img)
+    assert b.step_count == 7
+    b.step(5)
+    got = b.populations()
+    assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+    assert np.array_equal(a.radial()["sum_uy"], b.radial()["sum_uy"])
+
+
+def test_uninitialised_step_is_an_error(mcg):
+    case = mcg.jet_case(16, 16, 16, jet_radius=3.0)
+    s = mcg.Solver(case)
+    with pytest.raises(mcg.McgError):
+        s.step(1)

@@ -1,0 +1,312 @@
+"""Pins of the oracle's full step (CPU only).
+
+Each test checks a property that the reference's algorithm fixes
+independently of how the oracle is coded: fixed points, conservation laws,
+the write set of the in-place push (quirks Q1/Q2), hand-computed boundary
+values, the lattice viscosity nu = cs^2 (tau - 1/2), exactness of the colour
+gradient on linear fields, and brute-force recomputation of the statistics.
+"""
+import numpy as np
+import pytest
+
+import multicomponent_color_gradient_amd as m
+from oracle import oracle as orc
+from oracle.oracle import Oracle
+
+CS2 = 1.0 / 3.0
+INNER = (slice(1, -1),) * 3
+
+
+def c_vectors():
+    L = orc.lattice()
+    return L, np.stack([L["cx"], L["cy"], L["cz"]], 1)
+
+
+@pytest.mark.parametrize("ymode", [0, 1])
+def test_rest_state_is_a_fixed_point(ymode):
+    case = m.jet_case(14, 12, 10, jet_radius=0.0)
+    case.y_mode = ymode
+    o = Oracle(case)
+    o.init_jet()
+    f0 = o.populations()
+    o.step(5)
+    f1 = o.populations()
+    sl = (slice(None),) + INNER
+    # exact up to rounding: sum_i w_i and sum_i w_i (c_x^2 - cs^2) are 1 and 0 only to an ulp
+    assert np.max(np.abs(f0[0][sl] - f1[0][sl])) < 1e-15
+    assert np.max(np.abs(f0[1][sl] - f1[1][sl])) == 0.0
+
+
+def test_periodic_bubble_conserves_colour_mass_without_perturbation():
+    # A = 0: streaming, regularised BGK and recolouring conserve each colour and momentum
+    case = m.periodic_bubble_case(18, radius=4.5, tau=0.8, A=1e-3, beta=0.7)
+    case.A_override = 0.0
+    case.sigma = 0.0
+    o = Oracle(case)
+    o.init_bubble_tanh()
+    o.step(1)
+    s0 = o.global_sums()
+    o.step(40)
+    s1 = o.global_sums()
+    assert abs(s1[0] - s0[0]) <= 1e-12 * s0[0]
+    assert abs(s1[1] - s0[1]) <= 1e-12 * s0[1]
+    assert np.all(np.abs(s1[2:]) <= 1e-12)
+
+
+def test_perturbation_mass_leak_at_weak_gradients():
+    """The cos^2 rule is cut at |F| <= 1e-4 (forces.cuh:61-64) but -B_i is not
+    (collisionOperators.cuh:127), so an interface node (I > 1e-4) with a weak
+    gradient changes its mass by sum_i Omega2_i = -(A/2)|F| sum_i B_i = -(A/6)|F|,
+    shared between the colours as aR, aB.  Everywhere else mass is conserved."""
+    A = 1e-3
+    case = m.periodic_bubble_case(24, radius=5.0, tau=0.8, A=A, beta=0.7)
+    o = Oracle(case)
+    o.init_bubble_tanh()
+    o.step(5)
+    mo = o.moments()
+    s0 = o.global_sums()
+    pred_r = pred_b = 0.0
+    n_weak = 0
+    for z in range(1, 23):
+        for y in range(1, 23):
+            for x in range(1, 23):
+                rr, rb = mo["rhor"][z, y, x], mo["rhob"][z, y, x]
+                rho = rr + rb
+                I = 4 * rr * rb / rho / rho
+                if I <= 1e-4:
+                    continue
+                aF = float(np.linalg.norm(o.gradient(x, y, z)))
+                if aF <= 1e-4:
+                    n_weak += 1
+                    pred_r += rr / rho * (-(A / 6) * aF)
+                    pred_b += rb / rho * (-(A / 6) * aF)
+    o.step(1)
+    s1 = o.global_sums()
+    assert n_weak > 0
+    assert abs((s1[0] - s0[0]) - pred_r) <= 1e-12 + 1e-6 * abs(pred_r)
+    assert abs((s1[1] - s0[1]) - pred_b) <= 1e-12 + 1e-6 * abs(pred_b)
+
+
+def predicted_stale(case):
+    """Interior slots the reference never writes after initialisation:
+    Q1 - c_y=+1 slots of plane y=1 whose ghost source (x-cx, 0, z-cz) is not a
+    jet node of the inner frame (jet/boundary_conditions.cuh:102-105,137);
+    Q2 - c_y=-1 slots of plane y=NY-2 whose ghost source lies on the x/z frame
+    (neumann runs on inner columns only and writes without wrap, :286)."""
+    L, c = c_vectors()
+    NX, NY, NZ = case.NX, case.NY, case.NZ
+    stale = np.zeros((27, NZ, NY, NX), bool)
+    for i in range(27):
+        cx, cy, cz = c[i]
+        if cy == 0:
+            continue
+        y = 1 if cy == 1 else NY - 2
+        for z in range(1, NZ - 1):
+            for x in range(1, NX - 1):
+                xs, zs = x - cx, z - cz
+                inner = 1 <= xs <= NX - 2 and 1 <= zs <= NZ - 2
+                if cy == 1:
+                    jet = inner and (xs - case.jet_x0) ** 2 + (zs - case.jet_z0) ** 2 <= case.jet_radius ** 2
+                    stale[i, z, y, x] = not jet
+                else:
+                    stale[i, z, y, x] = not inner
+    return stale
+
+
+def test_write_set_quirks_q1_q2():
+    case = m.jet_case(20, 12, 20, jet_radius=7.0)
+    o = Oracle(case)
+    o.init_jet()
+    cr, cb = o.writeset()
+    sl = (slice(None),) + INNER
+    assert cr[sl].max() == 1 and cb[sl].max() == 1  # at most one writer per slot
+    zero = (cr == 0)[sl]
+    pred = predicted_stale(case)[sl]
+    assert np.array_equal(zero, pred)
+    assert np.array_equal(cr, cb)
+    assert int(pred.sum()) > 0
+
+
+def test_stale_slots_keep_initial_values():
+    case = m.jet_case(20, 12, 20, jet_radius=5.0, Re=300.0)
+    o = Oracle(case)
+    o.init_jet()
+    o.step(30)
+    fr, fb = o.populations()
+    L, _ = c_vectors()
+    st = predicted_stale(case)
+    i, z, y, x = np.nonzero(st)
+    assert np.array_equal(fr[i, z, y, x], L["w"][i])
+    assert np.all(fb[i, z, y, x] == 0)
+
+
+def test_inlet_closed_form_after_one_step():
+    # at rest, Pi(y=1) = 0 and u(y=1) = 0 on the first step: jet-fed slots carry
+    # f_b = feq_i(1, (0,U,0)) = w_i (1 + 3U c_y + 3U^2 c_y^2 - 3U^2/2 ...) and f_r = 0
+    U = 0.05
+    case = m.jet_case(18, 10, 18, jet_radius=3.0, jet_velocity=U)
+    o = Oracle(case)
+    o.init_jet()
+    o.step(1)
+    fr, fb = o.populations()
+    L, c = c_vectors()
+    x0 = int(case.jet_x0)  # a jet column (8.5 - 0.5)
+    for i in np.where(c[:, 1] == 1)[0]:
+        cx, cz = c[i, 0], c[i, 2]
+        xt, zt = x0 + cx, x0 + cz
+        cu = U
+        usq = U * U
+        exp = L["w"][i] * (1 + cu / CS2 + cu * cu / (2 * CS2 ** 2) - usq / (2 * CS2) +
+                           cu * (cu * cu / (6 * CS2 ** 3) - usq / (2 * CS2 ** 2)))
+        assert abs(fb[i, zt, 1, xt] - exp) < 1e-15
+        assert fr[i, zt, 1, xt] == 0
+    # the ghost plane holds the jet mask
+    f = o.fields()
+    assert f["rhob"][x0, 0, x0] == 1 and f["rhor"][x0, 0, x0] == 0
+    assert f["rhor"][1, 0, 1] == 1 and f["rhob"][1, 0, 1] == 0
+
+
+def outlet_case():
+    return m.jet_case(10, 8, 10, jet_radius=0.0)
+
+
+def set_uy_column(o, case, uy_target):
+    """Populations at rest except a momentum of uy_target in column (4, NY-2, 5)."""
+    L, c = c_vectors()
+    fr, fb = o.populations()
+    x, y, z = 4, case.NY - 2, 5
+    fr[:, z, y, x] = [orc.feq(i, 1.0, 0.0, uy_target, 0.0) for i in range(27)]
+    o.set_populations(fr, fb)
+    return x, y, z
+
+
+@pytest.mark.parametrize("uy,uc", [(-0.3, 0.0), (0.4, 0.4), (1.6, 1.0)])
+def test_outlet_convective_clamp(uy, uc):
+    """phi_B <- phi_B - clamp01(u_y,F) (phi_B - phi_F) (jet/boundary_conditions.cuh:246-254)."""
+    case = outlet_case()
+    o = Oracle(case)
+    o.init_jet()
+    x, y, z = set_uy_column(o, case, uy)
+    o.step(1)
+    mo = o.moments()
+    f = o.fields()
+    uyF = mo["uy"][z, y, x]
+    assert abs(uyF - uy) < 1e-12  # the column's moments are what was set
+    assert abs(f["uy"][z, y + 1, x] - (0.0 - uc * (0.0 - uyF))) < 1e-14
+    assert abs(f["rhor"][z, y + 1, x] - (1.0 - uc * (1.0 - mo["rhor"][z, y, x]))) < 1e-14
+
+
+def test_shear_wave_viscosity():
+    """u_x = U sin(k y) decays as exp(-nu k^2 t), nu = cs^2 (tau - 1/2)."""
+    tau = 0.8
+    NY = 34  # period NY - 2 = 32
+    case = m.jet_case(6, NY, 6, jet_radius=0.0)
+    case.y_mode = 1
+    case.taur = case.taub = tau
+    o = Oracle(case)
+    o.init_jet()
+    L, c = c_vectors()
+    yy = np.arange(NY)
+    k = 2 * np.pi / (NY - 2)
+    U = 1e-3
+    ux = U * np.sin(k * (yy - 1))
+    fr, fb = o.populations()
+    for i in range(27):
+        fr[i] = np.array([orc.feq(i, 1.0, u, 0.0, 0.0) for u in ux])[None, :, None]
+    o.set_populations(fr, fb)
+    amps = []
+    steps = [20, 220]
+    done = 0
+    for t in steps:
+        o.step(t - done)
+        done = t
+        mo = o.moments()
+        prof = mo["ux"][3, 1:-1, 3]
+        amps.append(2 * np.mean(prof * np.sin(k * np.arange(NY - 2))))
+    rate = -np.log(amps[1] / amps[0]) / (steps[1] - steps[0])
+    nu = rate / k ** 2
+    assert abs(nu - CS2 * (tau - 0.5)) < 0.01 * CS2 * (tau - 0.5), nu
+
+
+def test_colour_gradient_exact_on_linear_field():
+    case = m.jet_case(12, 12, 12, jet_radius=0.0)
+    o = Oracle(case)
+    o.init_jet()
+    a = np.array([0.013, -0.021, 0.007])
+    z, y, x = np.meshgrid(np.arange(12), np.arange(12), np.arange(12), indexing="ij")
+    phi = 0.1 + a[0] * x + a[1] * y + a[2] * z
+    o.set_densities((1 + phi) / 2, (1 - phi) / 2)
+    for (px, py, pz) in [(5, 5, 5), (1, 1, 1), (10, 10, 10), (1, 6, 10)]:
+        assert np.allclose(o.gradient(px, py, pz), a, atol=1e-14)
+    o.set_densities(np.full(phi.shape, 0.6), np.full(phi.shape, 0.4))
+    assert np.allclose(o.gradient(5, 5, 5), 0, atol=1e-16)
+
+
+def test_moments_pass_recovers_constructed_state():
+    """Populations feq(rho,u) + fneqr(Pi,u) give back rho_r, rho_b, u, Pi."""
+    rng = np.random.default_rng(11)
+    case = m.jet_case(8, 8, 8, jet_radius=0.0)
+    case.y_mode = 1
+    o = Oracle(case)
+    o.init_jet()
+    rho_r, rho_b = 0.7, 0.25
+    u = np.array([0.02, -0.03, 0.01])
+    P = 1e-3 * rng.standard_normal(6)
+    g = np.array([orc.feq(i, rho_r + rho_b, *u) + orc.fneqr(i, P, *u) for i in range(27)])
+    a = rho_r / (rho_r + rho_b)
+    fr = np.broadcast_to((a * g)[:, None, None, None], (27,) + o.shape).copy()
+    fb = np.broadcast_to(((1 - a) * g)[:, None, None, None], (27,) + o.shape).copy()
+    o.set_populations(fr, fb)
+    o.step(1)
+    mo = o.moments()
+    c = (3, 4, 4)
+    assert abs(mo["rhor"][c] - rho_r) < 1e-14 and abs(mo["rhob"][c] - rho_b) < 1e-14
+    assert np.allclose([mo["ux"][c], mo["uy"][c], mo["uz"][c]], u, atol=1e-15)
+    got = [mo[k][c] for k in ("pxx", "pxy", "pyy", "pyz", "pzz", "pxz")]
+    assert np.allclose(got, P, atol=1e-15)
+
+
+def test_radial_statistics_brute_force():
+    case = m.jet_case(22, 14, 18, jet_radius=4.0, Re=400.0)
+    case.NSTATS_START_STEP = 2
+    case.NSTATS_SAMPLE = 3
+    o = Oracle(case)
+    o.init_jet()
+    NR = case.NX // 2
+    acc = {k: np.zeros((case.NY, NR)) for k in ("uy", "uy2", "ur", "ur2", "uruy", "phi")}
+    cnt = np.zeros((case.NY, NR), np.uint64)
+    zz, xx = np.meshgrid(np.arange(1, case.NZ - 1), np.arange(1, case.NX - 1), indexing="ij")
+    dx, dz = xx - case.jet_x0, zz - case.jet_z0
+    r = np.sqrt(dx * dx + dz * dz)
+    rb = r.astype(np.int64)
+    ok = rb < NR
+    for step in range(11):
+        o.step(1)
+        if step >= 2 and (step - 2) % 3 == 0:
+            mo = o.moments()
+            for y in range(1, case.NY - 1):
+                uy = mo["uy"][1:-1, y, 1:-1]
+                ur = np.where(r > 1e-12, (mo["ux"][1:-1, y, 1:-1] * dx + mo["uz"][1:-1, y, 1:-1] * dz) / np.maximum(r, 1e-300), 0)
+                ph = (mo["rhor"][1:-1, y, 1:-1] - mo["rhob"][1:-1, y, 1:-1]) / (mo["rhor"][1:-1, y, 1:-1] + mo["rhob"][1:-1, y, 1:-1])
+                for name, v in (("uy", uy), ("uy2", uy * uy), ("ur", ur), ("ur2", ur * ur), ("uruy", ur * uy), ("phi", ph)):
+                    np.add.at(acc[name][y], rb[ok], v[ok])
+                np.add.at(cnt[y], rb[ok], 1)
+    got = o.radial()
+    assert np.array_equal(got["count"], cnt)
+    info = o.stats_info()
+    assert info == {"started": True, "start_step": 2, "samples": 3}
+    assert int(cnt.sum()) == int(ok.sum()) * (case.NY - 2) * 3
+    for name in acc:
+        assert np.allclose(got["sum_" + name], acc[name], rtol=1e-12, atol=1e-15), name
+
+
+def test_tke_direct_sum():
+    case = m.jet_case(16, 14, 16, jet_radius=3.0, Re=400.0)
+    o = Oracle(case)
+    o.init_jet()
+    o.step(9)
+    f = o.fields()
+    ke = 0.5 * (f["ux"] ** 2 + f["uy"] ** 2 + f["uz"] ** 2)[INNER].sum()
+    assert abs(o.tke() - ke) <= 1e-13 * ke
+    s = o.global_sums()
+    assert abs(s[0] - f["rhor"][INNER].sum()) < 1e-10
