@@ -220,6 +220,48 @@ int mcg_time_steps(mcg_solver *s, int nsteps, double *ms);
 /* Bytes of device memory held by the solver. */
 int mcg_device_bytes(const mcg_solver *s, uint64_t *nbytes);
 
+/* ---- output files (the reference's layout; host code, no GPU needed) --- */
+
+/* Create <root>/JET_VTK/Re<Re>_We<We>/{mean_profiles,vti_data} and return the
+ * three paths (make_case_folder_name / ensure_case_directories,
+ * meanF/meanFieldsRT.cuh:36-69).  Any output pointer may be NULL. */
+int mcg_output_dirs(const mcg_case *c, const char *root, char *case_dir, char *mean_dir, char *vti_dir, size_t cap);
+
+/* case_info.txt (write_case_metadata, meanF/meanFieldsRT.cuh:93-118). */
+int mcg_write_case_info(const mcg_case *c, const char *case_dir);
+
+/* VTK XML ImageData file of rho_r, rho_b, rho, u over the full grid from host
+ * arrays (write_vti, io/save_data.cuh:137-200). */
+int mcg_write_vti_arrays(const char *path, int32_t NX, int32_t NY, int32_t NZ, const float *rhor, const float *rhob,
+                         const float *ux, const float *uy, const float *uz);
+
+/* radial_sum_*.bin, radial_count.bin and radial_profile_metadata.txt from
+ * host arrays (write_radial_profile_outputs / write_radial_profile_metadata,
+ * meanF/meanFieldsRT.cuh:229-364).  sum_phi / sum_phi2 may be NULL. */
+int mcg_write_radial_files(const mcg_case *c, const char *mean_dir, const double *sum_uy, const double *sum_uy2,
+                           const double *sum_ur, const double *sum_ur2, const double *sum_uruy, const uint64_t *count,
+                           const double *sum_phi, const double *sum_phi2, int32_t started, int64_t start_step,
+                           int64_t samples);
+
+/* Running TKE average and delta (update_tke_average kernels.cu:154-166,
+ * update_tke_state meanF/meanFieldsRT.cuh:191-227) in FP32.
+ * state = {tke_avg, tke_avg_prev, first_sample_flag}; initialise to {0,0,1}.
+ * out = {tke_total, tke_avg, |delta|} as appended to the tke_*.bin files. */
+int mcg_tke_update(float state[3], float tke_total, int64_t step, int32_t noutput, float out[3]);
+
+/* Truncate / append one sample to tke_total.bin, tke_avg.bin, tke_diff.bin
+ * (open_mean_fields_files / write_tke_outputs, meanF/meanFieldsRT.cuh:73-91,366-375). */
+int mcg_tke_files_reset(const char *mean_dir);
+int mcg_append_tke(const char *mean_dir, const float vals[3]);
+
+/* Solver-level writers: copy the state and call the writers above
+ * (write_vti_step_device io/save_data.cuh:203-215 writes lbm_%08d.vti). */
+int mcg_write_vti(mcg_solver *s, const char *path);
+int mcg_write_vti_step(mcg_solver *s, const char *vti_dir, int64_t step);
+int mcg_write_radial(mcg_solver *s, const char *mean_dir);
+
+const char *mcg_io_last_error(void);
+
 /* ---- misc ------------------------------------------------------------- */
 
 const char *mcg_last_error(void);

@@ -136,6 +136,18 @@ SIGNATURES = [
     ("mcg_checkpoint_load", c_int, [_PSOLVER, c_void_p, c_uint64]),
     ("mcg_time_steps", c_int, [_PSOLVER, c_int, _D]),
     ("mcg_device_bytes", c_int, [_PSOLVER, POINTER(c_uint64)]),
+    ("mcg_output_dirs", c_int, [POINTER(McgCase), c_char_p, c_char_p, c_char_p, c_char_p, c_size_t]),
+    ("mcg_write_case_info", c_int, [POINTER(McgCase), c_char_p]),
+    ("mcg_write_vti_arrays", c_int, [c_char_p, c_int32, c_int32, c_int32, _F, _F, _F, _F, _F]),
+    ("mcg_write_radial_files", c_int, [POINTER(McgCase), c_char_p, _D, _D, _D, _D, _D, POINTER(c_uint64), _D, _D,
+                                       c_int32, c_int64, c_int64]),
+    ("mcg_tke_update", c_int, [_F, c_float, c_int64, c_int32, _F]),
+    ("mcg_tke_files_reset", c_int, [c_char_p]),
+    ("mcg_append_tke", c_int, [c_char_p, _F]),
+    ("mcg_write_vti", c_int, [_PSOLVER, c_char_p]),
+    ("mcg_write_vti_step", c_int, [_PSOLVER, c_char_p, c_int64]),
+    ("mcg_write_radial", c_int, [_PSOLVER, c_char_p]),
+    ("mcg_io_last_error", c_char_p, []),
     ("mcg_last_error", c_char_p, []),
     ("mcg_version", c_char_p, []),
 ]

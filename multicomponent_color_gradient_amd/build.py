@@ -27,13 +27,14 @@ HIP_FLAGS = [
 
 
 def sources():
-    return [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith((".hip", ".hpp"))]
+    return [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith((".hip", ".hpp", ".cpp"))]
 
 
 def build(verbose: bool = False) -> str:
     """Compile the library (always recompiles) and return its path."""
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "api.hip"), "-o", tmp]
+    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "api.hip"),
+           os.path.join(CSRC, "io.cpp"), "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -44,7 +45,24 @@ def build(verbose: bool = False) -> str:
     if verbose and msgs:
         print("\n".join(msgs))
     os.replace(tmp, OUT)
+    build_driver(verbose)
     return OUT
+
+
+DRIVER = os.path.join(HERE, "mcg_jet")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The C++ run driver (main.cu's loop) linked against the library."""
+    cxx = os.environ.get("CXX", shutil.which("g++") or "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "driver.cpp"),
+           "-L", HERE, "-lmcg_hip", "-Wl,-rpath,$ORIGIN", "-o", DRIVER]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + res.stdout + res.stderr)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/mcg.h"
 #include "device.hpp"
@@ -255,7 +256,7 @@ int one_step(mcg_solver *s) {
     } else {
         DevBufs Bf = B;
         bool have_moments = false;
-        if (do_stats && !fused_has_stats()) {
+        if (do_stats && !fused_has_stats(s->P)) {
             // statistics from the moments of this step, LDS-privatised histogram
             int rc = launch_moments(s, s->cur);
             if (rc) return rc;
@@ -664,6 +665,42 @@ int mcg_time_steps(mcg_solver *s, int nsteps, double *ms) {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *ms = double(t);
+    return MCG_OK;
+}
+
+int mcg_write_vti(mcg_solver *s, const char *path) {
+    if (int rc = check_ready(s)) return rc;
+    if (!path) return fail(MCG_E_INVALID, "null path");
+    const size_t N = s->N();
+    std::vector<float> h(5 * N);
+    if (int rc = mcg_copy_fields(s, h.data(), h.data() + N, h.data() + 2 * N, h.data() + 3 * N, h.data() + 4 * N)) return rc;
+    const int rc = mcg_write_vti_arrays(path, s->c.NX, s->c.NY, s->c.NZ, h.data(), h.data() + N, h.data() + 2 * N,
+                                        h.data() + 3 * N, h.data() + 4 * N);
+    if (rc) return fail(rc, mcg_io_last_error());
+    return MCG_OK;
+}
+
+int mcg_write_vti_step(mcg_solver *s, const char *vti_dir, int64_t step) {
+    if (!vti_dir) return fail(MCG_E_INVALID, "null directory");
+    char name[64];
+    std::snprintf(name, sizeof(name), "/lbm_%08lld.vti", (long long)step);
+    return mcg_write_vti(s, (std::string(vti_dir) + name).c_str());
+}
+
+int mcg_write_radial(mcg_solver *s, const char *mean_dir) {
+    if (int rc = check_ready(s)) return rc;
+    const size_t nb = s->bins();
+    std::vector<double> h(7 * nb);
+    std::vector<uint64_t> cnt(nb);
+    if (s->started && s->samples > 0) {
+        if (int rc = mcg_copy_radial(s, h.data(), h.data() + nb, h.data() + 2 * nb, h.data() + 3 * nb, h.data() + 4 * nb,
+                                     cnt.data(), h.data() + 5 * nb, h.data() + 6 * nb))
+            return rc;
+    }
+    const int rc = mcg_write_radial_files(&s->c, mean_dir, h.data(), h.data() + nb, h.data() + 2 * nb, h.data() + 3 * nb,
+                                          h.data() + 4 * nb, cnt.data(), h.data() + 5 * nb, h.data() + 6 * nb,
+                                          s->started ? 1 : 0, s->start_step, s->samples);
+    if (rc) return fail(rc, mcg_io_last_error());
     return MCG_OK;
 }
 

@@ -8,6 +8,7 @@ utilities/indexing.cuh:10-13) and ``(27, NZ, NY, NX)`` for populations
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -145,6 +146,16 @@ class Solver:
         fb = np.ascontiguousarray(fb, dtype=np.float32).reshape((27,) + self.shape)
         f = ctypes.c_float
         check(self.lib.mcg_set_populations(self._h, as_ptr(fr, f), as_ptr(fb, f)))
+
+    # ---- output files (io/save_data.cuh, meanF/meanFieldsRT.cuh)
+    def write_vti(self, path: str) -> None:
+        check(self.lib.mcg_write_vti(self._h, os.fsencode(path)))
+
+    def write_vti_step(self, vti_dir: str, step: int) -> None:
+        check(self.lib.mcg_write_vti_step(self._h, os.fsencode(vti_dir), int(step)))
+
+    def write_radial(self, mean_dir: str) -> None:
+        check(self.lib.mcg_write_radial(self._h, os.fsencode(mean_dir)))
 
     # ---- checkpoint / restart
     def checkpoint(self) -> np.ndarray:
