@@ -164,7 +164,8 @@ def load_library(path: str | None = None):
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    # MCG_LIB_PATH selects another build of the same library (e.g. a tuning variant)
+    p = path or os.environ.get("MCG_LIB_PATH") or LIB_PATH
     if not os.path.exists(p):
         raise McgError(MCG_E_STATE, f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
     lib = ctypes.CDLL(p)

@@ -21,6 +21,9 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_*_f32 (fp32 arithmetic stays scalar per lane)
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-munsafe-fp-atomics",
+    # no implicit multiply-add contraction: every kernel rounds the same expression the same way,
+    # so the fused and two-pass paths give identical bits
+    "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]

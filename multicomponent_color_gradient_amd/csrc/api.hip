@@ -41,6 +41,16 @@ int fail(int code, const std::string &msg) {
 
 constexpr int kInitJet = 0, kInitBubble = 1, kInitBubbleTanh = 2;
 
+// launch KERNEL<storage> for the run-time storage mode
+#define MCG_DISPATCH_ST(st, KERNEL, ...)                                                 \
+    do {                                                                                 \
+        switch (st) {                                                                    \
+            case MCG_STORE_FP16: hipLaunchKernelGGL((KERNEL<MCG_STORE_FP16>), __VA_ARGS__); break; \
+            case MCG_STORE_BF16: hipLaunchKernelGGL((KERNEL<MCG_STORE_BF16>), __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((KERNEL<MCG_STORE_FP32>), __VA_ARGS__); break;   \
+        }                                                                                \
+    } while (0)
+
 }  // namespace
 
 struct mcg_solver {
@@ -51,7 +61,7 @@ struct mcg_solver {
     hipStream_t stream = nullptr;
     int device = 0;
 
-    float *f[2] = {nullptr, nullptr};
+    void *f[2] = {nullptr, nullptr};
     float *bin[2] = {nullptr, nullptr};
     float *bout[2] = {nullptr, nullptr};
     float *gB[2] = {nullptr, nullptr};
@@ -77,7 +87,7 @@ struct mcg_solver {
 
     size_t N() const { return size_t(c.NX) * c.NY * c.NZ; }
     size_t nxz() const { return size_t(c.NX) * c.NZ; }
-    size_t pop_bytes() const { return size_t(2 * Q) * P.plane * sizeof(float); }
+    size_t pop_bytes() const { return size_t(storage == MCG_STORE_FP32 ? 2 * Q : Q) * P.plane * sizeof(float); }
     size_t bnd_bytes() const { return size_t(18) * nxz() * sizeof(float); }
     size_t gB_bytes() const { return size_t(5) * nxz() * sizeof(float); }
     size_t bins() const { return size_t(c.NY) * P.NR; }
@@ -131,6 +141,7 @@ void fill_params(const mcg_case &c, int storage, DevParams &P) {
     P.IY = c.NY - 2;
     P.IZ = c.NZ - 2;
     P.plane = uint32_t(size_t(P.PX) * P.IY * P.IZ);
+    P.rowstride = uint32_t(P.PX * (storage == MCG_STORE_FP32 ? 2 * Q : Q));
     P.y_mode = c.y_mode;
     P.wrap_gradient = c.wrap_gradient;
     P.visc_mode = c.viscosity_mode;
@@ -167,8 +178,8 @@ int check_ready(const mcg_solver *s) {
 }
 
 int launch_moments(mcg_solver *s, int which) {
-    hipLaunchKernelGGL(k_moments, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->f[which], s->bin[which],
-                       s->bout[which], s->mom);
+    MCG_DISPATCH_ST(s->storage, k_moments, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->f[which], s->bin[which],
+                    s->bout[which], s->mom);
     LAUNCH_CHECK("k_moments");
     return MCG_OK;
 }
@@ -210,7 +221,7 @@ int do_init(mcg_solver *s, int kind) {
         HIP_TRY(hipMemsetAsync(s->bout[b], 0, s->bnd_bytes(), s->stream));
     }
     HIP_TRY(hipMemsetAsync(s->mom, 0, size_t(11) * s->P.plane * sizeof(float), s->stream));
-    hipLaunchKernelGGL(k_init_pull, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, kind, s->geo, s->f[0], s->bin[0],
+    MCG_DISPATCH_ST(s->storage, k_init_pull, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, kind, s->geo, s->f[0], s->bin[0],
                        s->bout[0]);
     LAUNCH_CHECK("k_init_pull");
     hipLaunchKernelGGL(k_init_ghost, s->grid_full(), dim3(64, 4, 1), 0, s->stream, s->P, kind, s->geo, s->grr, s->grb,
@@ -250,7 +261,7 @@ int one_step(mcg_solver *s) {
                                s->stats, s->count);
             LAUNCH_CHECK("k_radial");
         }
-        hipLaunchKernelGGL(k_collide, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, B);
+        MCG_DISPATCH_ST(s->storage, k_collide, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, B);
         LAUNCH_CHECK("k_collide");
         s->moments_valid = true;
     } else {
@@ -339,7 +350,7 @@ int mcg_case_validate(const mcg_case *c) {
     if (c->NX > 2048) return fail(MCG_E_INVALID, "NX > 2048 (radial bins NX/2 must fit the LDS histogram)");
     const size_t px = size_t((c->NX - 2 + 63) / 64 * 64);
     const size_t plane = px * size_t(c->NY - 2) * size_t(c->NZ - 2);
-    if (plane * 27 >= (size_t(1) << 32)) return fail(MCG_E_INVALID, "grid too large for 32-bit plane offsets");
+    if (plane * 54 >= (size_t(1) << 32)) return fail(MCG_E_INVALID, "grid too large for 32-bit population offsets");
     if (c->y_mode != MCG_Y_JET && c->y_mode != MCG_Y_PERIODIC) return fail(MCG_E_INVALID, "unknown y_mode");
     if (c->viscosity_mode != MCG_VISC_SINGLE && c->viscosity_mode != MCG_VISC_INTERFACE)
         return fail(MCG_E_INVALID, "unknown viscosity_mode");
@@ -355,7 +366,8 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
     *out = nullptr;
     int rc = mcg_case_validate(c);
     if (rc) return rc;
-    if (storage != MCG_STORE_FP32) return fail(MCG_E_UNSUPPORTED, "storage mode not implemented");
+    if (storage != MCG_STORE_FP32 && storage != MCG_STORE_FP16 && storage != MCG_STORE_BF16)
+        return fail(MCG_E_INVALID, "unknown storage mode");
     mcg_solver *s = new (std::nothrow) mcg_solver();
     if (!s) return fail(MCG_E_NOMEM, "host allocation failed");
     s->c = *c;
@@ -397,7 +409,7 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
 int mcg_destroy(mcg_solver *s) {
     if (!s) return MCG_OK;
     for (int b = 0; b < 2; ++b) {
-        for (float *p : {s->f[b], s->bin[b], s->bout[b], s->gB[b]})
+        for (void *p : {s->f[b], (void *)s->bin[b], (void *)s->bout[b], (void *)s->gB[b]})
             if (p) (void)hipFree(p);
     }
     for (void *p : {(void *)s->grr, (void *)s->grb, (void *)s->mom, (void *)s->stats, (void *)s->count,
@@ -528,7 +540,7 @@ int mcg_copy_populations(mcg_solver *s, float *fr, float *fb) {
     const size_t n = size_t(Q) * s->N();
     if (int rc = s->ensure_staging(2 * n * sizeof(float))) return rc;
     HIP_TRY(hipMemsetAsync(s->staging, 0, 2 * n * sizeof(float), s->stream));
-    hipLaunchKernelGGL(k_gather_pops, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->f[s->cur], s->bin[s->cur],
+    MCG_DISPATCH_ST(s->storage, k_gather_pops, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->f[s->cur], s->bin[s->cur],
                        s->bout[s->cur], s->staging, s->staging + n);
     LAUNCH_CHECK("k_gather_pops");
     if (fr) HIP_TRY(hipMemcpyAsync(fr, s->staging, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
@@ -544,7 +556,7 @@ int mcg_set_populations(mcg_solver *s, const float *fr, const float *fb) {
     if (int rc = s->ensure_staging(2 * n * sizeof(float))) return rc;
     HIP_TRY(hipMemcpyAsync(s->staging, fr, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->staging + n, fb, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_scatter_pops, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->f[s->cur], s->bin[s->cur],
+    MCG_DISPATCH_ST(s->storage, k_scatter_pops, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->f[s->cur], s->bin[s->cur],
                        s->bout[s->cur], s->staging, s->staging + n);
     LAUNCH_CHECK("k_scatter_pops");
     // entries the boundary never rewrites must agree in both copies

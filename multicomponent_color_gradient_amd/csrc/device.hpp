@@ -40,7 +40,8 @@ struct DevParams {
     int NX, NY, NZ;
     int PX;           // padded interior row pitch (elements)
     int IY, IZ;       // NY-2, NZ-2
-    uint32_t plane;   // PX*IY*IZ
+    uint32_t plane;   // PX*IY*IZ, elements of one moment plane
+    uint32_t rowstride;  // populations: elements of one (y,z) row block, PX * (54 or 27)
     int y_mode, wrap_gradient, visc_mode;
     int sponge_ystart, sponge_yend;
     float nu, sponge_K;
@@ -56,8 +57,8 @@ struct DevParams {
 };
 
 struct DevBufs {
-    const float *f_src;
-    float *f_dst;
+    const void *f_src;     // populations read this step (Store<storage> layout)
+    void *f_dst;           // populations written this step
     const float *in_src;
     float *in_dst;
     const float *out_src;
@@ -72,7 +73,14 @@ struct DevBufs {
     int do_stats;
 };
 
+// population row-block index of interior node (x,y,z): the 54 (FP32) or 27
+// (16-bit) direction rows of one (y,z) row lie next to each other, so one
+// wavefront's gather touches a few contiguous 1 KB rows instead of 54 planes
 __device__ __forceinline__ uint32_t pidx(const DevParams &P, int x, int y, int z) {
+    return uint32_t(x - 1) + P.rowstride * (uint32_t(y - 1) + uint32_t(P.IY) * uint32_t(z - 1));
+}
+// moment-plane index of interior node (x,y,z)
+__device__ __forceinline__ uint32_t midx(const DevParams &P, int x, int y, int z) {
     return uint32_t(x - 1) + uint32_t(P.PX) * (uint32_t(y - 1) + uint32_t(P.IY) * uint32_t(z - 1));
 }
 __device__ __forceinline__ size_t gidx(const DevParams &P, int x, int y, int z) {
@@ -111,17 +119,112 @@ __device__ __forceinline__ float A_node(const DevParams &P, float rr, float rb) 
     return P.sig4 / tau;
 }
 
-// Location of the population that streams into interior node (x,y,z) along
-// direction I: (base pointer, element offset of colour r, stride to colour b).
-struct SrcRef {
-    const float *base;
+// ---- population storage ------------------------------------------------
+// Direction I of colour c sits at element (c*27 + I)*stride + off of the
+// node's row block (stride = PX, off = pidx).
+// FP32: two rows per direction ([colour*27 + i]).
+// FP16 / BF16: one 32-bit word per direction and node: the low half holds
+// t/w_i - 1 with t = f_r + f_b (the offset of the two-colour population from
+// its lattice weight, scaled by 1/w_i), the high half holds the minority
+// colour m/w_i with its sign bit naming the minority colour (+: blue is the
+// minority, -: red is).  Pure phases (m = 0) are therefore exact, and the
+// arithmetic stays FP32.  Populations are assumed non-negative (they are in a
+// stable run; the watchdog flags non-finite states).
+constexpr float invw(int i) { return 1.0f / w(i); }
+
+__device__ __forceinline__ uint32_t f32_bits(float v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ float bits_f32(uint32_t v) { return __builtin_bit_cast(float, v); }
+
+// float -> bf16 bits, round to nearest even (NaN kept quiet)
+__device__ __forceinline__ uint32_t bf16_bits(float v) {
+    const uint32_t u = f32_bits(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+template <int ST>
+struct Store;
+
+template <>
+struct Store<MCG_STORE_FP32> {
+    static constexpr int planes = 2 * Q;
+    template <int I>
+    __device__ __forceinline__ static void load(const void *f, uint32_t stride, uint32_t off, float &fr, float &fb) {
+        const float *p = static_cast<const float *>(f);
+        fr = p[uint32_t(I) * stride + off];
+        fb = p[uint32_t(Q + I) * stride + off];
+    }
+    template <int I>
+    __device__ __forceinline__ static void store(void *f, uint32_t stride, uint32_t off, float fr, float fb) {
+        float *p = static_cast<float *>(f);
+        p[uint32_t(I) * stride + off] = fr;
+        p[uint32_t(Q + I) * stride + off] = fb;
+    }
+};
+
+template <int ST>
+struct Store16 {
+    static constexpr int planes = Q;
+    __device__ __forceinline__ static uint32_t half_bits(float v) {
+        if constexpr (ST == MCG_STORE_FP16) {
+            return uint32_t(__builtin_bit_cast(uint16_t, static_cast<_Float16>(v)));
+        } else {
+            return bf16_bits(v);
+        }
+    }
+    __device__ __forceinline__ static float half_value(uint32_t h) {
+        if constexpr (ST == MCG_STORE_FP16) {
+            return float(__builtin_bit_cast(_Float16, uint16_t(h)));
+        } else {
+            return bits_f32(h << 16);
+        }
+    }
+    template <int I>
+    __device__ __forceinline__ static void decode(uint32_t v, float &fr, float &fb) {
+        const float a = half_value(v & 0xffffu);
+        const uint32_t hb = v >> 16;
+        const float t = (a + 1.0f) * w(I);
+        const float m = half_value(hb & 0x7fffu) * w(I);
+        if (hb & 0x8000u) {
+            fr = m;
+            fb = t - m;
+        } else {
+            fb = m;
+            fr = t - m;
+        }
+    }
+    template <int I>
+    __device__ __forceinline__ static uint32_t encode(float fr, float fb) {
+        const float t = fr + fb;
+        const float a = t * invw(I) - 1.0f;
+        const bool red_major = fr >= fb;
+        const float m = fabsf(red_major ? fb : fr) * invw(I);
+        return half_bits(a) | ((half_bits(m) | (red_major ? 0u : 0x8000u)) << 16);
+    }
+    template <int I>
+    __device__ __forceinline__ static void load(const void *f, uint32_t stride, uint32_t off, float &fr, float &fb) {
+        decode<I>(static_cast<const uint32_t *>(f)[uint32_t(I) * stride + off], fr, fb);
+    }
+    template <int I>
+    __device__ __forceinline__ static void store(void *f, uint32_t stride, uint32_t off, float fr, float fb) {
+        static_cast<uint32_t *>(f)[uint32_t(I) * stride + off] = encode<I>(fr, fb);
+    }
+};
+template <>
+struct Store<MCG_STORE_FP16> : Store16<MCG_STORE_FP16> {};
+template <>
+struct Store<MCG_STORE_BF16> : Store16<MCG_STORE_BF16> {};
+
+// Source of the population that streams into interior node (x,y,z) along
+// direction I: kind 0 = interior node (off = its plane offset), 1 / 2 = the
+// inlet / outlet source buffer (off = element of colour r, colour b at +NX*NZ).
+struct Src {
+    int kind;
     size_t off;
-    size_t cstride;
 };
 
 template <int I>
-__device__ __forceinline__ SrcRef source_of(const DevParams &P, const float *f, const float *bin, const float *bout,
-                                            int x, int y, int z) {
+__device__ __forceinline__ Src source_of(const DevParams &P, int x, int y, int z) {
     int xs = x - kCX[I];
     int ys = y - kCY[I];
     int zs = z - kCZ[I];
@@ -132,7 +235,7 @@ __device__ __forceinline__ SrcRef source_of(const DevParams &P, const float *f, 
                 // ghost-plane sources are not wrapped (jet/boundary_conditions.cuh:137,286)
                 const size_t nxz = size_t(P.NX) * size_t(P.NZ);
                 const size_t off = size_t(slot_of(I)) * 2 * nxz + size_t(zs) * P.NX + size_t(xs);
-                return SrcRef{kCY[I] > 0 ? bin : bout, off, nxz};
+                return Src{kCY[I] > 0 ? 1 : 2, off};
             }
         } else {
             ys = wrapN(ys, P.NY);
@@ -140,27 +243,45 @@ __device__ __forceinline__ SrcRef source_of(const DevParams &P, const float *f, 
     }
     xs = wrapN(xs, P.NX);
     zs = wrapN(zs, P.NZ);
-    const size_t off = size_t(I) * P.plane + pidx(P, xs, ys, zs);
-    return SrcRef{f, off, size_t(Q) * P.plane};
+    return Src{0, size_t(pidx(P, xs, ys, zs))};
 }
 
-template <int I>
-__device__ __forceinline__ void gather(const DevParams &P, const float *__restrict__ f, const float *__restrict__ bin,
+template <int ST, int I>
+__device__ __forceinline__ void gather(const DevParams &P, const void *__restrict__ f, const float *__restrict__ bin,
                                        const float *__restrict__ bout, int x, int y, int z, float &fr, float &fb) {
-    const SrcRef s = source_of<I>(P, f, bin, bout, x, y, z);
-    fr = s.base[s.off];
-    fb = s.base[s.off + s.cstride];
+    const Src s = source_of<I>(P, x, y, z);
+    if (s.kind == 0) {
+        Store<ST>::template load<I>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb);
+    } else {
+        const float *b = s.kind == 1 ? bin : bout;
+        fr = b[s.off];
+        fb = b[s.off + size_t(P.NX) * P.NZ];
+    }
+}
+
+template <int ST, int I>
+__device__ __forceinline__ void scatter(const DevParams &P, void *f, float *bin, float *bout, int x, int y, int z,
+                                        float fr, float fb) {
+    const Src s = source_of<I>(P, x, y, z);
+    if (s.kind == 0) {
+        Store<ST>::template store<I>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb);
+    } else {
+        float *b = s.kind == 1 ? bin : bout;
+        b[s.off] = fr;
+        b[s.off + size_t(P.NX) * P.NZ] = fb;
+    }
 }
 
 // moments of interior node (x,y,z) from the gathered populations (lbm.cuh:82-165)
-__device__ __forceinline__ Moments gather_moments(const DevParams &P, const float *__restrict__ f,
+template <int ST>
+__device__ __forceinline__ Moments gather_moments(const DevParams &P, const void *__restrict__ f,
                                                   const float *__restrict__ bin, const float *__restrict__ bout,
                                                   int x, int y, int z) {
     MomentAcc acc;
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
         float a, b;
-        gather<I>(P, f, bin, bout, x, y, z, a, b);
+        gather<ST, I>(P, f, bin, bout, x, y, z, a, b);
         acc.add<I>(a, b);
     });
     return acc.finish();

@@ -28,12 +28,13 @@ namespace mcg {
 
 // density-based psi of interior node (x,y,z), summed in the order of
 // lbm.cuh:106-115 so that it equals psi of the moments pass bit for bit
+template <int ST>
 __device__ __forceinline__ float gathered_psi(const DevParams &P, const DevBufs &B, int x, int y, int z) {
     float sr = 0.f, sb = 0.f;
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
         float a, b;
-        gather<I>(P, B.f_src, B.in_src, B.out_src, x, y, z, a, b);
+        gather<ST, I>(P, B.f_src, B.in_src, B.out_src, x, y, z, a, b);
         sr += a;
         sb += b;
     });
@@ -41,11 +42,14 @@ __device__ __forceinline__ float gathered_psi(const DevParams &P, const DevBufs 
 }
 
 // psi as the colour gradient sees it at any grid position; 0 off the grid
-__device__ __forceinline__ float psi_at(const DevParams &P, const DevBufs &B, int x, int y, int z) {
+// out of line: called for the halo ring and ghost positions only, it keeps the
+// generic gather's address arithmetic out of the main loop's register budget
+template <int ST>
+__device__ __noinline__ float psi_at(const DevParams &P, const DevBufs &B, int x, int y, int z) {
     if (x < 0 || x >= P.NX || y < 0 || y >= P.NY || z < 0 || z >= P.NZ) return 0.0f;
     int xn, yn, zn;
     grad_neighbor(P, x, y, z, xn, yn, zn);
-    if (interior(P, xn, yn, zn)) return gathered_psi(P, B, xn, yn, zn);
+    if (interior(P, xn, yn, zn)) return gathered_psi<ST>(P, B, xn, yn, zn);
     return ghost_psi(P, B, xn, yn, zn);
 }
 
@@ -53,7 +57,7 @@ __device__ __forceinline__ float psi_at(const DevParams &P, const DevBufs &B, in
 // per-lane offset corrections (same addresses as source_of()).
 struct FastSrc {
     uint32_t p;
-    int wxp, wxm, wyp, wym, wzp, wzm;
+    uint32_t wxp, wxm, wyp, wym, wzp, wzm;  // modulo-2^32 offset corrections
     int o;
     bool yin, yout;
 };
@@ -61,22 +65,22 @@ struct FastSrc {
 __device__ __forceinline__ FastSrc make_src(const DevParams &P, int x, int y, int z) {
     FastSrc s;
     s.p = pidx(P, x, y, z);
-    const int dy = P.PX, dz = P.PX * P.IY;
-    s.wxp = (x == 1) ? (P.NX - 2) : 0;
-    s.wxm = (x == P.NX - 2) ? -(P.NX - 2) : 0;
-    s.wzp = (z == 1) ? (P.NZ - 2) * dz : 0;
-    s.wzm = (z == P.NZ - 2) ? -(P.NZ - 2) * dz : 0;
+    const uint32_t dy = P.rowstride, dz = P.rowstride * uint32_t(P.IY);
+    s.wxp = (x == 1) ? uint32_t(P.NX - 2) : 0u;
+    s.wxm = (x == P.NX - 2) ? 0u - uint32_t(P.NX - 2) : 0u;
+    s.wzp = (z == 1) ? uint32_t(P.NZ - 2) * dz : 0u;
+    s.wzm = (z == P.NZ - 2) ? 0u - uint32_t(P.NZ - 2) * dz : 0u;
     const bool jet = P.y_mode == MCG_Y_JET;
-    s.wyp = (!jet && y == 1) ? (P.NY - 2) * dy : 0;
-    s.wym = (!jet && y == P.NY - 2) ? -(P.NY - 2) * dy : 0;
+    s.wyp = (!jet && y == 1) ? uint32_t(P.NY - 2) * dy : 0u;
+    s.wym = (!jet && y == P.NY - 2) ? 0u - uint32_t(P.NY - 2) * dy : 0u;
     s.yin = jet && y == 1;
     s.yout = jet && y == P.NY - 2;
     s.o = z * P.NX + x;
     return s;
 }
 
-template <int I>
-__device__ __forceinline__ void fast_load(const DevParams &P, const float *__restrict__ f, const float *__restrict__ bin,
+template <int ST, int I>
+__device__ __forceinline__ void fast_load(const DevParams &P, const void *__restrict__ f, const float *__restrict__ bin,
                                           const float *__restrict__ bout, const FastSrc &s, float &fr, float &fb) {
     constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
     const size_t nxz = size_t(P.NX) * P.NZ;
@@ -98,24 +102,25 @@ __device__ __forceinline__ void fast_load(const DevParams &P, const float *__res
             return;
         }
     }
-    int off = int(s.p) - cx - cy * P.PX - cz * (P.PX * P.IY);
+    // unsigned arithmetic: intermediate values may wrap, the final offset is in range
+    uint32_t off = s.p - uint32_t(cx) - uint32_t(cy) * P.rowstride - uint32_t(cz) * (P.rowstride * uint32_t(P.IY));
     if constexpr (cx > 0) off += s.wxp;
     if constexpr (cx < 0) off += s.wxm;
     if constexpr (cy > 0) off += s.wyp;
     if constexpr (cy < 0) off += s.wym;
     if constexpr (cz > 0) off += s.wzp;
     if constexpr (cz < 0) off += s.wzm;
-    fr = f[size_t(I) * P.plane + uint32_t(off)];
-    fb = f[size_t(Q + I) * P.plane + uint32_t(off)];
+    Store<ST>::template load<I>(f, uint32_t(P.PX), off, fr, fb);
 }
 
+template <int ST>
 __device__ __forceinline__ Moments fast_moments(const DevParams &P, const DevBufs &B, int x, int y, int z) {
     const FastSrc s = make_src(P, x, y, z);
     MomentAcc acc;
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
         float a, b;
-        fast_load<I>(P, B.f_src, B.in_src, B.out_src, s, a, b);
+        fast_load<ST, I>(P, B.f_src, B.in_src, B.out_src, s, a, b);
         acc.add<I>(a, b);
     });
     return acc.finish();
@@ -125,8 +130,13 @@ struct FusedGrid {
     int nxt, nyb, nzs, zlen, nwg;
 };
 
-template <int TY, bool STATS>
-__global__ __launch_bounds__(64 * TY) void k_fused(DevParams P, DevBufs B, FusedGrid G) {
+#ifndef MCG_FUSED_WAVES_PER_EU
+#define MCG_FUSED_WAVES_PER_EU 1
+#endif
+
+template <int TY, bool STATS, int ST>
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(MCG_FUSED_WAVES_PER_EU)))
+void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     constexpr int H = TY + 2;
     constexpr int W = 66;
     __shared__ float phs[4][H][W];
@@ -165,19 +175,19 @@ __global__ __launch_bounds__(64 * TY) void k_fused(DevParams P, DevBufs B, Fused
                 r = (h - 128) >> 1;
                 c = ((h - 128) & 1) ? W - 1 : 0;
             }
-            phs[zz & 3][r][c] = psi_at(P, B, x0 - 1 + c, y0 - 1 + r, zz);
+            phs[zz & 3][r][c] = psi_at<ST>(P, B, x0 - 1 + c, y0 - 1 + r, zz);
         }
     };
 
     // prologue: psi of plane zb-1, moments and psi of plane zb
-    phs[(zb - 1) & 3][ty + 1][tx + 1] = psi_at(P, B, x, y, zb - 1);
+    phs[(zb - 1) & 3][ty + 1][tx + 1] = psi_at<ST>(P, B, x, y, zb - 1);
     halo(zb - 1);
     Moments cur{};
     if (valid) {
-        cur = fast_moments(P, B, x, y, zb);
+        cur = fast_moments<ST>(P, B, x, y, zb);
         phs[zb & 3][ty + 1][tx + 1] = psi(cur.rr, cur.rb);
     } else {
-        phs[zb & 3][ty + 1][tx + 1] = psi_at(P, B, x, y, zb);
+        phs[zb & 3][ty + 1][tx + 1] = psi_at<ST>(P, B, x, y, zb);
     }
     halo(zb);
 
@@ -186,10 +196,10 @@ __global__ __launch_bounds__(64 * TY) void k_fused(DevParams P, DevBufs B, Fused
         const int zl = z + 1;
         Moments nxt{};
         if (valid && zl <= ze) {
-            nxt = fast_moments(P, B, x, y, zl);
+            nxt = fast_moments<ST>(P, B, x, y, zl);
             phs[zl & 3][ty + 1][tx + 1] = psi(nxt.rr, nxt.rb);
         } else {
-            phs[zl & 3][ty + 1][tx + 1] = psi_at(P, B, x, y, zl);
+            phs[zl & 3][ty + 1][tx + 1] = psi_at<ST>(P, B, x, y, zl);
         }
         halo(zl);
         __syncthreads();
@@ -205,8 +215,7 @@ __global__ __launch_bounds__(64 * TY) void k_fused(DevParams P, DevBufs B, Fused
                 constexpr int I = decltype(Ic)::value;
                 float a, b;
                 collide_dir<I>(n, a, b);
-                B.f_dst[size_t(I) * P.plane + p] = a;
-                B.f_dst[size_t(Q + I) * P.plane + p] = b;
+                Store<ST>::template store<I>(B.f_dst, uint32_t(P.PX), p, a, b);
             });
             if (P.y_mode == MCG_Y_JET) {
                 if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, cur, x, z);
@@ -274,16 +283,25 @@ inline int fused_zsegs(const DevParams &P) {
 
 inline bool fused_has_stats(const DevParams &P) { return size_t(kFusedTY) * P.NR * 8 * sizeof(double) <= 65536; }
 
-inline int launch_fused(const DevParams &P, const DevBufs &B, hipStream_t stream) {
+template <int ST>
+inline int launch_fused_st(const DevParams &P, const DevBufs &B, hipStream_t stream) {
     const FusedGrid G = fused_grid(P, fused_zsegs(P));
     const dim3 block(64, kFusedTY, 1);
     if (B.do_stats) {
         const size_t shm = size_t(kFusedTY) * P.NR * 8 * sizeof(double);
-        hipLaunchKernelGGL((k_fused<kFusedTY, true>), dim3(G.nwg), block, shm, stream, P, B, G);
+        hipLaunchKernelGGL((k_fused<kFusedTY, true, ST>), dim3(G.nwg), block, shm, stream, P, B, G);
     } else {
-        hipLaunchKernelGGL((k_fused<kFusedTY, false>), dim3(G.nwg), block, 0, stream, P, B, G);
+        hipLaunchKernelGGL((k_fused<kFusedTY, false, ST>), dim3(G.nwg), block, 0, stream, P, B, G);
     }
     return int(hipGetLastError());
+}
+
+inline int launch_fused(const DevParams &P, const DevBufs &B, hipStream_t stream) {
+    switch (P.storage) {
+        case MCG_STORE_FP16: return launch_fused_st<MCG_STORE_FP16>(P, B, stream);
+        case MCG_STORE_BF16: return launch_fused_st<MCG_STORE_BF16>(P, B, stream);
+        default: return launch_fused_st<MCG_STORE_FP32>(P, B, stream);
+    }
 }
 
 }  // namespace mcg

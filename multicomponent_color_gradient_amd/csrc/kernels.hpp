@@ -49,16 +49,14 @@ struct InitGeo {
 
 // Populations f_i = w_i rho_k at every interior node, stored at the pull
 // source of each slot (initial state of jetDensity / bubble, kernels.cu:13-42).
-__global__ __launch_bounds__(256) void k_init_pull(DevParams P, int kind, InitGeo geo, float *f, float *bin, float *bout) {
+template <int ST>
+__global__ __launch_bounds__(256) void k_init_pull(DevParams P, int kind, InitGeo geo, void *f, float *bin, float *bout) {
     MCG_INTERIOR_XYZ(P, x, y, z)
     float rr, rb;
     init_rho(P, kind, geo.g, x, y, z, rr, rb);
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
-        const SrcRef s = source_of<I>(P, f, bin, bout, x, y, z);
-        float *base = const_cast<float *>(s.base);
-        base[s.off] = w(I) * rr;
-        base[s.off + s.cstride] = w(I) * rb;
+        scatter<ST, I>(P, f, bin, bout, x, y, z, w(I) * rr, w(I) * rb);
     });
 }
 
@@ -100,11 +98,12 @@ __global__ __launch_bounds__(256) void k_inlet_ghost_reset(DevParams P, float *g
 // ---------------------------------------------------------------- two-pass step
 
 // Pass 1: gather + moments (Mfields, kernels.cu:60-76, lbm.cuh:82-165)
-__global__ __launch_bounds__(256) void k_moments(DevParams P, const float *__restrict__ f, const float *__restrict__ bin,
+template <int ST>
+__global__ __launch_bounds__(256) void k_moments(DevParams P, const void *__restrict__ f, const float *__restrict__ bin,
                                                  const float *__restrict__ bout, float *__restrict__ mom) {
     MCG_INTERIOR_XYZ(P, x, y, z)
-    const Moments m = gather_moments(P, f, bin, bout, x, y, z);
-    const uint32_t p = pidx(P, x, y, z);
+    const Moments m = gather_moments<ST>(P, f, bin, bout, x, y, z);
+    const uint32_t p = midx(P, x, y, z);
     const size_t pl = P.plane;
     mom[0 * pl + p] = m.rr;
     mom[1 * pl + p] = m.rb;
@@ -140,16 +139,17 @@ __device__ __forceinline__ Moments load_moments(const DevParams &P, const float 
 // stored at the node (ColliStream, kernels.cu:78-94, lbm.cuh:184-293), plus
 // the inlet (inlet, kernels.cu:98-114) and outlet (neumann, kernels.cu:116-132)
 // populations of the columns this node feeds.
+template <int ST>
 __global__ __launch_bounds__(256) void k_collide(DevParams P, DevBufs B) {
     MCG_INTERIOR_XYZ(P, x, y, z)
     const uint32_t p = pidx(P, x, y, z);
-    const Moments m = load_moments(P, B.mom, p);
+    const Moments m = load_moments(P, B.mom, midx(P, x, y, z));
     auto phi = [&](auto Ic) -> float {
         constexpr int I = decltype(Ic)::value;
         int xn, yn, zn;
         grad_neighbor(P, x + kCX[I], y + kCY[I], z + kCZ[I], xn, yn, zn);
         if (interior(P, xn, yn, zn)) {
-            const uint32_t q = pidx(P, xn, yn, zn);
+            const uint32_t q = midx(P, xn, yn, zn);
             return psi(B.mom[q], B.mom[size_t(P.plane) + q]);
         }
         return ghost_psi(P, B, xn, yn, zn);
@@ -159,8 +159,7 @@ __global__ __launch_bounds__(256) void k_collide(DevParams P, DevBufs B) {
         constexpr int I = decltype(Ic)::value;
         float a, b;
         collide_dir<I>(n, a, b);
-        B.f_dst[size_t(I) * P.plane + p] = a;
-        B.f_dst[size_t(Q + I) * P.plane + p] = b;
+        Store<ST>::template store<I>(B.f_dst, uint32_t(P.PX), p, a, b);
     });
     if (P.y_mode == MCG_Y_JET) {
         if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z);
@@ -220,7 +219,7 @@ __global__ __launch_bounds__(256) void k_radial(DevParams P, const float *__rest
     for (int t = threadIdx.x; t < ix * nz; t += blockDim.x) {
         const int x = 1 + t % ix;
         const int z = z0 + t / ix;
-        const uint32_t p = pidx(P, x, y, z);
+        const uint32_t p = midx(P, x, y, z);
         Moments m;
         m.rr = mom[p];
         m.rb = mom[size_t(P.plane) + p];
@@ -252,7 +251,7 @@ __global__ __launch_bounds__(256) void k_reduce_partials(DevParams P, const floa
         const int x = 1 + int(t % ix);
         const int y = 1 + int((t / ix) % iy);
         const int z = 1 + int(t / (ix * iy));
-        const uint32_t p = pidx(P, x, y, z);
+        const uint32_t p = midx(P, x, y, z);
         const float rr = mom[p], rb = mom[size_t(P.plane) + p];
         const float vx = mom[2 * size_t(P.plane) + p], vy = mom[3 * size_t(P.plane) + p], vz = mom[4 * size_t(P.plane) + p];
         const float ke = 0.5f * (vx * vx + vy * vy + vz * vz);  // meanF/meanFields.cuh:10-19
@@ -309,7 +308,7 @@ __global__ __launch_bounds__(256) void k_export_fields(DevParams P, const float 
     const size_t g = gidx(P, x, y, z);
     float v[5];
     if (interior(P, x, y, z)) {
-        const uint32_t p = pidx(P, x, y, z);
+        const uint32_t p = midx(P, x, y, z);
         for (int k = 0; k < 5; ++k) v[k] = zero_interior ? 0.0f : mom[k * size_t(P.plane) + p];
     } else if (P.y_mode == MCG_Y_JET && y == P.NY - 1 && x >= 1 && x <= P.NX - 2 && z >= 1 && z <= P.NZ - 2) {
         const size_t nxz = size_t(P.NX) * P.NZ;
@@ -324,7 +323,8 @@ __global__ __launch_bounds__(256) void k_export_fields(DevParams P, const float 
 }
 
 // populations the next step reads, reference layout i*Ncells + id
-__global__ __launch_bounds__(256) void k_gather_pops(DevParams P, const float *__restrict__ f, const float *__restrict__ bin,
+template <int ST>
+__global__ __launch_bounds__(256) void k_gather_pops(DevParams P, const void *__restrict__ f, const float *__restrict__ bin,
                                                      const float *__restrict__ bout, float *out_r, float *out_b) {
     MCG_INTERIOR_XYZ(P, x, y, z)
     const size_t N = size_t(P.NX) * P.NY * P.NZ;
@@ -332,23 +332,21 @@ __global__ __launch_bounds__(256) void k_gather_pops(DevParams P, const float *_
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
         float a, b;
-        gather<I>(P, f, bin, bout, x, y, z, a, b);
+        gather<ST, I>(P, f, bin, bout, x, y, z, a, b);
         out_r[size_t(I) * N + g] = a;
         out_b[size_t(I) * N + g] = b;
     });
 }
 
-__global__ __launch_bounds__(256) void k_scatter_pops(DevParams P, float *f, float *bin, float *bout,
+template <int ST>
+__global__ __launch_bounds__(256) void k_scatter_pops(DevParams P, void *f, float *bin, float *bout,
                                                       const float *__restrict__ in_r, const float *__restrict__ in_b) {
     MCG_INTERIOR_XYZ(P, x, y, z)
     const size_t N = size_t(P.NX) * P.NY * P.NZ;
     const size_t g = gidx(P, x, y, z);
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
-        const SrcRef s = source_of<I>(P, f, bin, bout, x, y, z);
-        float *base = const_cast<float *>(s.base);
-        base[s.off] = in_r[size_t(I) * N + g];
-        base[s.off + s.cstride] = in_b[size_t(I) * N + g];
+        scatter<ST, I>(P, f, bin, bout, x, y, z, in_r[size_t(I) * N + g], in_b[size_t(I) * N + g]);
     });
 }
 

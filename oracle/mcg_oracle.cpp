@@ -615,22 +615,30 @@ struct Oracle final : OracleBase {
         if (ns) *ns = samples_;
     }
     // compute_total_tke, kernels.cu:136-152 and meanF/meanFields.cuh:10-19 (FP64 sum)
+    // Sums are formed per z plane and then added in plane order, so the
+    // result does not depend on the OpenMP thread count or timing.
     double tke() override {
-        double t = 0;
-#pragma omp parallel for reduction(+ : t) schedule(static)
-        for (int z = 1; z <= NZ - 2; ++z)
+        std::vector<double> part(NZ, 0.0);
+#pragma omp parallel for schedule(static)
+        for (int z = 1; z <= NZ - 2; ++z) {
+            double t = 0;
             for (int y = 1; y <= NY - 2; ++y)
                 for (int x = 1; x <= NX - 2; ++x) {
                     const size_t id = idx(x, y, z);
                     const double a = ux[id], b = uy[id], d = uz[id];
                     t += 0.5 * (a * a + b * b + d * d);
                 }
+            part[z] = t;
+        }
+        double t = 0;
+        for (int z = 1; z <= NZ - 2; ++z) t += part[z];
         return t;
     }
     void global_sums(double *o) override {
-        double a = 0, b = 0, jx = 0, jy = 0, jz = 0;
-#pragma omp parallel for reduction(+ : a, b, jx, jy, jz) schedule(static)
-        for (int z = 1; z <= NZ - 2; ++z)
+        std::vector<double> part(size_t(NZ) * 5, 0.0);
+#pragma omp parallel for schedule(static)
+        for (int z = 1; z <= NZ - 2; ++z) {
+            double a = 0, b = 0, jx = 0, jy = 0, jz = 0;
             for (int y = 1; y <= NY - 2; ++y)
                 for (int x = 1; x <= NX - 2; ++x) {
                     const size_t id = idx(x, y, z);
@@ -641,7 +649,12 @@ struct Oracle final : OracleBase {
                     jy += r * uy[id];
                     jz += r * uz[id];
                 }
-        o[0] = a; o[1] = b; o[2] = jx; o[3] = jy; o[4] = jz;
+            double *q = &part[size_t(z) * 5];
+            q[0] = a; q[1] = b; q[2] = jx; q[3] = jy; q[4] = jz;
+        }
+        for (int k = 0; k < 5; ++k) o[k] = 0.0;
+        for (int z = 1; z <= NZ - 2; ++z)
+            for (int k = 0; k < 5; ++k) o[k] += part[size_t(z) * 5 + k];
     }
     void gradient(int x, int y, int z, double F[3]) override {
         T a, b, d;

@@ -1,0 +1,85 @@
+"""16-bit population storage (FP16 / BF16 offsets, FP32 arithmetic).
+
+Encoding per (node, direction): t/w_i - 1 with t = f_r + f_b, and the
+minority colour m/w_i whose sign bit names the minority colour.
+Round-trip bounds (round to nearest, |t/w - 1| < 1/4 here):
+  FP16: |d t| <= 2^-14 w_i (1 + |t|/w_i 2^-2),  |d m| <= 2^-11 m
+  BF16: |d t| <= 2^-11 w_i,                      |d m| <= 2^-8 m
+"""
+import numpy as np
+import pytest
+
+from conftest import INNER
+
+pytestmark = pytest.mark.gpu
+
+BOUNDS = {"fp16": (2.0 ** -14 * 1.3, 2.0 ** -11), "bf16": (2.0 ** -11, 2.0 ** -8)}
+
+
+def weights():
+    from oracle.oracle import lattice
+
+    return lattice()["w"][:, None, None, None]
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_round_trip_bounds_and_pure_phases(mcg, storage):
+    case = mcg.jet_case(20, 16, 18, jet_radius=3.0)
+    s = mcg.Solver(case, storage=storage)
+    s.init_jet()
+    rng = np.random.default_rng(3)
+    w = weights()
+    shape = s.shape
+    frac = rng.random(shape)
+    frac[:, :, :6] = 0.0  # pure red
+    frac[:, :, 6:9] = 1.0  # pure blue
+    t = w * (1 + 0.2 * (rng.random((27,) + shape) - 0.5))
+    fr = (t * (1 - frac)).astype(np.float32)
+    fb = (t * frac).astype(np.float32)
+    s.set_populations(fr, fb)
+    gr, gb = s.populations()
+    sl = (slice(None),) + INNER
+    bt, bm = BOUNDS[storage]
+    tt = (fr + fb).astype(np.float64)
+    mm = np.minimum(fr, fb).astype(np.float64)
+    dt = np.abs((gr + gb).astype(np.float64) - tt)[sl]
+    assert np.all(dt <= bt * (w * np.ones(shape))[sl] + 2e-7 * tt[sl])
+    dm = np.abs(np.minimum(gr, gb).astype(np.float64) - mm)[sl]
+    assert np.all(dm <= bm * mm[sl] + 1e-12)
+    # pure phases stay exactly pure
+    assert np.all(gb[(slice(None),) + (slice(1, -1), slice(1, -1), slice(1, 6))] == 0)
+    assert np.all(gr[(slice(None),) + (slice(1, -1), slice(1, -1), slice(6, 9))] == 0)
+
+
+@pytest.mark.parametrize("storage,tol_u", [("fp16", 2e-4), ("bf16", 2e-3)])
+def test_jet_16bit_close_to_fp64_oracle(mcg, storage, tol_u):
+    """Velocity error after 100 jet steps, against the FP64 oracle: the FP32 run
+    is within 1e-5; 16-bit storage adds its rounding (|d u| <= tol_u)."""
+    from oracle.oracle import Oracle
+
+    case = mcg.jet_case(30, 40, 30, jet_radius=5.0, Re=500.0)
+    s = mcg.Solver(case, storage=storage)
+    s.init_jet()
+    s.step(100)
+    o = Oracle(case)
+    o.init_jet()
+    o.step(100)
+    g, r = s.fields(), o.fields()
+    for k in ("ux", "uy", "uz"):
+        e = float(np.max(np.abs(g[k][INNER] - r[k][INNER])))
+        assert e <= tol_u, f"{storage} {k}: {e:.3e}"
+    e = float(np.max(np.abs(g["rhob"][INNER] - r["rhob"][INNER])))
+    assert e <= 20 * tol_u
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_16bit_fused_equals_twopass(mcg, storage):
+    case = mcg.jet_case(34, 30, 30, jet_radius=5.0, Re=500.0)
+    a = mcg.Solver(case, storage=storage, kernel="fused")
+    b = mcg.Solver(case, storage=storage, kernel="twopass")
+    a.init_jet()
+    b.init_jet()
+    a.step(20)
+    b.step(20)
+    pa, pb = a.populations(), b.populations()
+    assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1])
