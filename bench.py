@@ -62,6 +62,28 @@ def load_traffic(workload: str, storage: str):
     return d.get("bytes_per_launch")
 
 
+def copy_probe(nbytes: int, reps: int = 3) -> None:
+    """Device-to-device hipMemcpy of nbytes, reps times: the runtime's blit copy kernel
+    reads and writes exactly nbytes per launch."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("copy probe: hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)):
+            raise RuntimeError("copy probe: hipMemset failed")
+        for _ in range(reps):
+            if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+                raise RuntimeError("copy probe: hipMemcpy failed")
+        if hip.hipDeviceSynchronize():
+            raise RuntimeError("copy probe: hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
 def cpu_baseline(case, init: str, budget_s: float):
     """FP64 OpenMP oracle on the same grid: 1 warm-up step, then steps until budget_s."""
     from oracle import oracle as orc
@@ -101,6 +123,9 @@ def main():
     ap.add_argument("--kernel", default="fused")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed steps, copy a device buffer of this size 3 times with hipMemcpy "
+                         "(a plain copy kernel of known size that calibrates rocprofv3's FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -131,6 +156,8 @@ def main():
     achieved = bpl * interior / (avg_ms / 1e3) / 1e9
     traffic = load_traffic(args.workload, args.storage)
     s.close()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib << 20)
 
     out = {
         "metric": "MLUPS (fused step, % HBM roofline at 432 B/LU FP32)",

@@ -60,6 +60,8 @@ struct mcg_solver {
     int kernel = MCG_KERNEL_FUSED;
     hipStream_t stream = nullptr;
     int device = 0;
+    int num_cu = 256;
+    size_t fused_lds = 65536;  // LDS a fused-kernel workgroup may use (statistics histogram + psi ring)
 
     void *f[2] = {nullptr, nullptr};
     float *bin[2] = {nullptr, nullptr};
@@ -267,7 +269,7 @@ int one_step(mcg_solver *s) {
     } else {
         DevBufs Bf = B;
         bool have_moments = false;
-        if (do_stats && !fused_has_stats(s->P)) {
+        if (do_stats && !fused_has_stats(s->P, s->fused_lds)) {
             // statistics from the moments of this step, LDS-privatised histogram
             int rc = launch_moments(s, s->cur);
             if (rc) return rc;
@@ -278,7 +280,7 @@ int one_step(mcg_solver *s) {
             Bf.do_stats = 0;
             have_moments = true;
         }
-        const int rc = launch_fused(s->P, Bf, s->stream);
+        const int rc = launch_fused(s->P, Bf, s->stream, s->num_cu);
         if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
         s->moments_valid = have_moments;
     }
@@ -377,6 +379,24 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
     if (hipGetDevice(&s->device) != hipSuccess) {
         delete s;
         return fail(MCG_E_HIP, "hipGetDevice failed (no GPU?)");
+    }
+    if (hipDeviceGetAttribute(&s->num_cu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || s->num_cu <= 0)
+        s->num_cu = 256;
+    {
+        // histograms larger than the default 64 KiB need the kernel's dynamic-LDS limit raised
+        int lds = 0;
+        const size_t need = fused_stats_lds(s->P) + fused_static_lds();
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) == hipSuccess &&
+            size_t(lds) >= need && need > 65536) {
+            int rc = 0;
+            switch (storage) {
+                case MCG_STORE_FP16: rc = fused_allow_lds<MCG_STORE_FP16>(fused_stats_lds(s->P)); break;
+                case MCG_STORE_BF16: rc = fused_allow_lds<MCG_STORE_BF16>(fused_stats_lds(s->P)); break;
+                default: rc = fused_allow_lds<MCG_STORE_FP32>(fused_stats_lds(s->P)); break;
+            }
+            if (rc == 0) s->fused_lds = size_t(lds);
+            (void)hipGetLastError();
+        }
     }
     auto alloc = [&](void **p, size_t nb) -> bool {
         if (hipMalloc(p, nb) != hipSuccess) return false;

@@ -148,6 +148,20 @@ struct Store;
 template <>
 struct Store<MCG_STORE_FP32> {
     static constexpr int planes = 2 * Q;
+    // raw words of one node's gather: load_raw issues the loads, decode_raw
+    // turns the words of direction I into (f_r, f_b) later
+    static constexpr int kWords = 2;
+    template <int I>
+    __device__ __forceinline__ static void load_raw(const void *f, uint32_t stride, uint32_t off, uint32_t *w) {
+        const uint32_t *p = static_cast<const uint32_t *>(f);
+        w[0] = p[uint32_t(I) * stride + off];
+        w[1] = p[uint32_t(Q + I) * stride + off];
+    }
+    template <int I>
+    __device__ __forceinline__ static void decode_raw(const uint32_t *w, float &fr, float &fb) {
+        fr = __builtin_bit_cast(float, w[0]);
+        fb = __builtin_bit_cast(float, w[1]);
+    }
     template <int I>
     __device__ __forceinline__ static void load(const void *f, uint32_t stride, uint32_t off, float &fr, float &fb) {
         const float *p = static_cast<const float *>(f);
@@ -204,6 +218,15 @@ struct Store16 {
     template <int I>
     __device__ __forceinline__ static void load(const void *f, uint32_t stride, uint32_t off, float &fr, float &fb) {
         decode<I>(static_cast<const uint32_t *>(f)[uint32_t(I) * stride + off], fr, fb);
+    }
+    static constexpr int kWords = 1;
+    template <int I>
+    __device__ __forceinline__ static void load_raw(const void *f, uint32_t stride, uint32_t off, uint32_t *w) {
+        w[0] = static_cast<const uint32_t *>(f)[uint32_t(I) * stride + off];
+    }
+    template <int I>
+    __device__ __forceinline__ static void decode_raw(const uint32_t *w, float &fr, float &fb) {
+        decode<I>(w[0], fr, fb);
     }
     template <int I>
     __device__ __forceinline__ static void store(void *f, uint32_t stride, uint32_t off, float fr, float fb) {
@@ -314,27 +337,69 @@ __device__ __forceinline__ void grad_neighbor(const DevParams &P, int x, int y, 
     }
 }
 
-// Per-node collision set-up, lbm.cuh:193-250 and forces.cuh:92-105.
+// Per-node collision set-up, lbm.cuh:193-250 and forces.cuh:92-105; the
+// colour gradient is evaluated at interface nodes only, as the reference does.
 template <class PhiFn>
 __device__ __forceinline__ NodeColl node_setup(const DevParams &P, const Moments &m, int y, PhiFn &&phi) {
     NodeColl n;
-    n.m = m;
-    n.rT = m.rr + m.rb;
-    const float invrT = 1.0f / n.rT;
+    const float rT = m.rr + m.rb;
+    const float invrT = 1.0f / rT;
     n.aR = m.rr * invrT;
     n.aB = m.rb * invrT;
-    n.oms = 1.0f - omega_node(P, y, m.rr, m.rb);
+    const float oms = 1.0f - omega_node(P, y, m.rr, m.rb);
+    n.ux = m.ux;
+    n.uy = m.uy;
+    n.uz = m.uz;
+    const float usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
+    n.e0 = rT * (1.0f - usq * inv_2cs2);
+    n.e1 = rT * inv_2cs4;
+    n.o1 = rT * (inv_cs2 - usq * inv_2cs4);
+    n.o3 = rT * inv_6cs6;
+    n.q2 = oms * inv_2cs4;
+    n.q3 = oms * inv_2cs6;
+    n.pxx = m.pxx;
+    n.pyy = m.pyy;
+    n.pzz = m.pzz;
+    n.dxy = 2.0f * m.pxy;
+    n.dyz = 2.0f * m.pyz;
+    n.dxz = 2.0f * m.pxz;
+    n.tr = -cs2 * (m.pxx + m.pyy + m.pzz);
+    n.txxy = m.pxx * m.uy + n.dxy * m.ux;
+    n.txxz = m.pxx * m.uz + n.dxz * m.ux;
+    n.txyy = m.pyy * m.ux + n.dxy * m.uy;
+    n.txzz = m.pzz * m.ux + n.dxz * m.uz;
+    n.tyyz = m.pyy * m.uz + n.dyz * m.uy;
+    n.tyzz = m.pzz * m.uy + n.dyz * m.uz;
+    n.txyz = n.dxy * m.uz + n.dxz * m.uy + n.dyz * m.ux;
+    n.Fx = n.Fy = n.Fz = n.s2 = n.sB = n.rc = 0.0f;
     const float I = 4.0f * m.rr * m.rb * invrT * invrT;
-    n.bulk = I <= 1.0e-4f;
-    n.Fx = n.Fy = n.Fz = n.absF = n.invF = n.A = n.coeff = 0.0f;
-    if (!n.bulk) {
+    if (!(I <= 1.0e-4f)) {
         color_gradient(phi, n.Fx, n.Fy, n.Fz);
-        n.A = A_node(P, m.rr, m.rb);
-        n.absF = sqrtf(n.Fx * n.Fx + n.Fy * n.Fy + n.Fz * n.Fz);
-        n.invF = 1.0f / n.absF;
-        n.coeff = P.beta * ((m.rr * m.rb) / (n.rT * n.rT));
+        const float hA = 0.5f * A_node(P, m.rr, m.rb);
+        const float absF = sqrtf(n.Fx * n.Fx + n.Fy * n.Fy + n.Fz * n.Fz);
+        n.sB = hA * absF;
+        if (!(absF <= 1.0e-4f)) {
+            const float invF = 1.0f / absF;
+            n.s2 = hA * invF;
+            n.rc = P.beta * ((m.rr * m.rb) / (rT * rT)) * rT * invF;
+        }
     }
     return n;
+}
+
+// All 54 post-collision populations of a node, handed to put(I, f_r, f_b).
+template <class PutFn>
+__device__ __forceinline__ void collide_node(const NodeColl &n, PutFn &&put) {
+    float r0, b0;
+    collide_rest(n, r0, b0);
+    put(std::integral_constant<int, 0>{}, r0, b0);
+    unroll<0, 13>([&](auto kc) {
+        constexpr int I = 2 * decltype(kc)::value + 1;
+        float rp, bp, rm, bm;
+        collide_pair<I>(n, rp, bp, rm, bm);
+        put(std::integral_constant<int, I>{}, rp, bp);
+        put(std::integral_constant<int, I + 1>{}, rm, bm);
+    });
 }
 
 // Inlet populations sent by ghost node (x,0,z) of a jet column, computed from

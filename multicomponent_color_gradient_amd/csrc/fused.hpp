@@ -26,31 +26,21 @@
 
 namespace mcg {
 
-// density-based psi of interior node (x,y,z), summed in the order of
-// lbm.cuh:106-115 so that it equals psi of the moments pass bit for bit
-template <int ST>
-__device__ __forceinline__ float gathered_psi(const DevParams &P, const DevBufs &B, int x, int y, int z) {
-    float sr = 0.f, sb = 0.f;
-    unroll<0, Q>([&](auto Ic) {
-        constexpr int I = decltype(Ic)::value;
-        float a, b;
-        gather<ST, I>(P, B.f_src, B.in_src, B.out_src, x, y, z, a, b);
-        sr += a;
-        sb += b;
-    });
-    return psi(sr, sb);
-}
-
-// psi as the colour gradient sees it at any grid position; 0 off the grid
-// out of line: called for the halo ring and ghost positions only, it keeps the
-// generic gather's address arithmetic out of the main loop's register budget
-template <int ST>
-__device__ __noinline__ float psi_at(const DevParams &P, const DevBufs &B, int x, int y, int z) {
-    if (x < 0 || x >= P.NX || y < 0 || y >= P.NY || z < 0 || z >= P.NZ) return 0.0f;
-    int xn, yn, zn;
-    grad_neighbor(P, x, y, z, xn, yn, zn);
-    if (interior(P, xn, yn, zn)) return gathered_psi<ST>(P, B, xn, yn, zn);
-    return ghost_psi(P, B, xn, yn, zn);
+// A copy of the run-time strides that the compiler cannot prove loop
+// invariant: without it every per-direction address offset is hoisted out of
+// the z loop into its own scalar register (~300 of them), which spills.
+__device__ __forceinline__ DevParams fresh_strides(const DevParams &P) {
+    DevParams Q2 = P;
+#define MCG_LAUNDER(f)                                  \
+    {                                                   \
+        uint32_t v_ = uint32_t(Q2.f);                   \
+        asm volatile("" : "+s"(v_));                    \
+        Q2.f = decltype(Q2.f)(v_);                      \
+    }
+    MCG_LAUNDER(NX) MCG_LAUNDER(NY) MCG_LAUNDER(NZ) MCG_LAUNDER(PX) MCG_LAUNDER(IY) MCG_LAUNDER(IZ)
+    MCG_LAUNDER(plane) MCG_LAUNDER(rowstride) MCG_LAUNDER(y_mode) MCG_LAUNDER(wrap_gradient)
+#undef MCG_LAUNDER
+    return Q2;
 }
 
 // Pull source of an interior node with the periodic wraps folded into
@@ -114,7 +104,8 @@ __device__ __forceinline__ void fast_load(const DevParams &P, const void *__rest
 }
 
 template <int ST>
-__device__ __forceinline__ Moments fast_moments(const DevParams &P, const DevBufs &B, int x, int y, int z) {
+__device__ __forceinline__ Moments fast_moments(const DevParams &P0, const DevBufs &B, int x, int y, int z) {
+    const DevParams P = fresh_strides(P0);
     const FastSrc s = make_src(P, x, y, z);
     MomentAcc acc;
     unroll<0, Q>([&](auto Ic) {
@@ -126,109 +117,188 @@ __device__ __forceinline__ Moments fast_moments(const DevParams &P, const DevBuf
     return acc.finish();
 }
 
+// psi of any grid position as the colour gradient sees it (0 off the grid,
+// the ghost value at ghost positions) and, for interior positions (after the
+// optional periodic wrap), the moments of the gathered populations.  The
+// colour densities are summed in the order of lbm.cuh:106-115 exactly as the
+// moments pass sums them, so psi here and psi from stored moments agree bit
+// for bit.  has_m tells whether m was written.
+template <int ST>
+__device__ __forceinline__ float gather_pos(const DevParams &P0, const DevBufs &B, int x, int y, int z, Moments &m,
+                                            bool &has_m) {
+    const DevParams P = fresh_strides(P0);
+    has_m = false;
+    if (x < 0 || x >= P.NX || y < 0 || y >= P.NY || z < 0 || z >= P.NZ) return 0.0f;
+    int xn, yn, zn;
+    grad_neighbor(P, x, y, z, xn, yn, zn);
+    if (!interior(P, xn, yn, zn)) return ghost_psi(P, B, xn, yn, zn);
+    m = fast_moments<ST>(P, B, xn, yn, zn);
+    has_m = true;
+    return psi(m.rr, m.rb);
+}
+
+// Split gather for software pipelining: raw_issue starts the 27 or 54 loads
+// of an interior node whose sources are all population rows (no inlet or
+// outlet source buffer), raw_reduce turns the words into moments later.  The
+// decode and the summation order are those of fast_moments, so the moments
+// are bitwise the same.
+template <int ST>
+struct RawGather {
+    uint32_t w[Q * Store<ST>::kWords];
+};
+
+template <int ST>
+__device__ __forceinline__ void raw_issue(const DevParams &P0, const DevBufs &B, int x, int y, int z, RawGather<ST> &r) {
+    const DevParams P = fresh_strides(P0);
+    const FastSrc s = make_src(P, x, y, z);
+    unroll<0, Q>([&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+        uint32_t off = s.p - uint32_t(cx) - uint32_t(cy) * P.rowstride - uint32_t(cz) * (P.rowstride * uint32_t(P.IY));
+        if constexpr (cx > 0) off += s.wxp;
+        if constexpr (cx < 0) off += s.wxm;
+        if constexpr (cy > 0) off += s.wyp;
+        if constexpr (cy < 0) off += s.wym;
+        if constexpr (cz > 0) off += s.wzp;
+        if constexpr (cz < 0) off += s.wzm;
+        Store<ST>::template load_raw<I>(B.f_src, uint32_t(P.PX), off, r.w + I * Store<ST>::kWords);
+    });
+}
+
+template <int ST>
+__device__ __forceinline__ Moments raw_reduce(const RawGather<ST> &r) {
+    MomentAcc acc;
+    unroll<0, Q>([&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        float a, b;
+        Store<ST>::template decode_raw<I>(r.w + I * Store<ST>::kWords, a, b);
+        acc.add<I>(a, b);
+    });
+    return acc.finish();
+}
+
 struct FusedGrid {
     int nxt, nyb, nzs, zlen, nwg;
 };
 
 #ifndef MCG_FUSED_WAVES_PER_EU
-#define MCG_FUSED_WAVES_PER_EU 1
+#define MCG_FUSED_WAVES_PER_EU 3
 #endif
+// core rows per workgroup of the fused kernel
+#ifndef MCG_FUSED_TY
+#define MCG_FUSED_TY 9
+#endif
+constexpr int kFusedTY = MCG_FUSED_TY;
+// TY core waves, two waves for the y-halo rows, one for the x-halo columns
+constexpr int kFusedWaves = kFusedTY + 3;
 
-template <int TY, bool STATS, int ST>
-__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(MCG_FUSED_WAVES_PER_EU)))
+// Tile of workgroup b when the hardware deals workgroups round-robin over the
+// 8 XCDs: XCD k gets a contiguous range of tiles, so tiles that share halo
+// rows and 128-byte edge lines run at the same time behind the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n >> 3, r = n & 7, k = b & 7;
+    return k * q + min(k, r) + (b >> 3);
+}
+
+// Workgroup = TY core waves (one y-row of 64 x-columns each) + 3 halo waves.
+// Every lane owns one gather position of the (TY+2) x 66 psi tile: core lanes
+// their node, halo wave TY the row y0-1, halo wave TY+1 the row y0+TY, halo
+// wave TY+2 (lanes < 2(TY+2)) the columns x0-1 and x0+64.  Iteration zi
+// gathers the lead plane zi+1 into the psi ring (core lanes keep the moments
+// in registers), passes one barrier, and collides plane zi; the first two
+// iterations only fill the ring and the moments of plane zb.  The gather and
+// the collision each appear once in the code, which keeps the loop inside the
+// instruction cache.
+template <bool STATS, int ST>
+__global__ __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_per_eu(MCG_FUSED_WAVES_PER_EU)))
 void k_fused(DevParams P, DevBufs B, FusedGrid G) {
+    constexpr int TY = kFusedTY;
     constexpr int H = TY + 2;
     constexpr int W = 66;
+    constexpr int NT = 64 * kFusedWaves;
     __shared__ float phs[4][H][W];
     extern __shared__ double hist[];  // STATS: [TY][NR][8], the last slot a u64 count
 
-    int t = int(blockIdx.x);
-    if ((G.nwg & 7) == 0) t = (t & 7) * (G.nwg >> 3) + (t >> 3);  // contiguous tiles per XCD
+    const int t = xcd_tile(int(blockIdx.x), G.nwg);
     const int xt = t % G.nxt;
     const int r1 = t / G.nxt;
     const int yb = r1 % G.nyb;
     const int zs = r1 / G.nyb;
-    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
-    const int lid = ty * 64 + tx;
+    const int tx = int(threadIdx.x), wv = int(threadIdx.y);
+    const int lid = wv * 64 + tx;
     const int x0 = 1 + xt * 64, y0 = 1 + yb * TY;
-    const int x = x0 + tx, y = y0 + ty;
-    const bool valid = x <= P.NX - 2 && y <= P.NY - 2;
     const int zb = 1 + zs * G.zlen;
     const int ze = min(P.NZ - 2, zb + G.zlen - 1);
+    const bool core = wv < TY;
+    const int x = x0 + tx, y = y0 + wv;
+    const bool node = core && x <= P.NX - 2 && y <= P.NY - 2;
+
+    // this lane's gather position in tile coordinates
+    int lr = wv + 1, lc = tx + 1;
+    bool active = true;
+    if (wv == TY) {
+        lr = 0;
+    } else if (wv == TY + 1) {
+        lr = H - 1;
+    } else if (wv == TY + 2) {
+        active = tx < 2 * H;
+        lr = tx >> 1;
+        lc = (tx & 1) ? W - 1 : 0;
+    }
+    const int gx = x0 - 1 + lc, gy = y0 - 1 + lr;
 
     if constexpr (STATS) {
-        for (int k = lid; k < TY * P.NR * 8; k += 64 * TY) hist[k] = 0.0;
+        for (int k = lid; k < TY * P.NR * 8; k += NT) hist[k] = 0.0;
     }
 
-    // halo ring of one plane: rows y0-1 and y0+TY (64 columns), columns x0-1 and x0+64 (H rows)
-    constexpr int NH = 128 + 2 * H;
-    auto halo = [&](int zz) {
-        for (int h = lid; h < NH; h += 64 * TY) {
-            int r, c;
-            if (h < 64) {
-                r = 0;
-                c = h + 1;
-            } else if (h < 128) {
-                r = H - 1;
-                c = h - 63;
-            } else {
-                r = (h - 128) >> 1;
-                c = ((h - 128) & 1) ? W - 1 : 0;
-            }
-            phs[zz & 3][r][c] = psi_at<ST>(P, B, x0 - 1 + c, y0 - 1 + r, zz);
-        }
-    };
+    // the lane's lead position after the optional periodic wrap, and whether
+    // its gather can be split around the collision (interior, no inlet or
+    // outlet source buffer among its sources)
+    int gxn = gx, gyn = gy, gzn_unused = 0;
+    bool on_grid = gx >= 0 && gx < P.NX && gy >= 0 && gy < P.NY;
+    if (on_grid) grad_neighbor(P, gx, gy, 1, gxn, gyn, gzn_unused);
+    const bool split_xy = active && on_grid && gxn >= 1 && gxn <= P.NX - 2 && gyn >= 1 && gyn <= P.NY - 2 &&
+                          !(P.y_mode == MCG_Y_JET && (gyn == 1 || gyn == P.NY - 2));
 
-    // prologue: psi of plane zb-1, moments and psi of plane zb
-    phs[(zb - 1) & 3][ty + 1][tx + 1] = psi_at<ST>(P, B, x, y, zb - 1);
-    halo(zb - 1);
-    Moments cur{};
-    if (valid) {
-        cur = fast_moments<ST>(P, B, x, y, zb);
-        phs[zb & 3][ty + 1][tx + 1] = psi(cur.rr, cur.rb);
-    } else {
-        phs[zb & 3][ty + 1][tx + 1] = psi_at<ST>(P, B, x, y, zb);
-    }
-    halo(zb);
-
-    for (int z = zb; z <= ze; ++z) {
-        // A: lead plane
-        const int zl = z + 1;
-        Moments nxt{};
-        if (valid && zl <= ze) {
-            nxt = fast_moments<ST>(P, B, x, y, zl);
-            phs[zl & 3][ty + 1][tx + 1] = psi(nxt.rr, nxt.rb);
-        } else {
-            phs[zl & 3][ty + 1][tx + 1] = psi_at<ST>(P, B, x, y, zl);
-        }
-        halo(zl);
-        __syncthreads();
-        // B: collide plane z
-        if (valid) {
+    Moments cur{}, nx1{}, nx2{};
+    RawGather<ST> raw;
+    // iteration zi collides plane zi (once zi >= zb) and gathers the lead plane zi+2;
+    // the three fill iterations gather planes zb-1, zb and zb+1
+    for (int zi = zb - 3; zi <= ze; ++zi) {
+        // the plane index passes through an opaque copy so that addresses are
+        // recomputed per plane instead of becoming loop-carried pointers
+        int z = zi;
+        asm volatile("" : "+s"(z));
+        const int zl = z + 2;
+        const bool lead = zl <= ze + 1;
+        int zln = zl;
+        if (P.wrap_gradient) zln = wrapN(zl, P.NZ);
+        const bool split = lead && split_xy && zln >= 1 && zln <= P.NZ - 2;
+        if (split) raw_issue<ST>(P, B, gxn, gyn, zln, raw);
+        if (node && z >= zb) {
+            const Moments &m = cur;
             auto phi = [&](auto Ic) -> float {
                 constexpr int I = decltype(Ic)::value;
-                return phs[(z + kCZ[I]) & 3][ty + 1 + kCY[I]][tx + 1 + kCX[I]];
+                return phs[(z + kCZ[I]) & 3][wv + 1 + kCY[I]][tx + 1 + kCX[I]];
             };
-            const NodeColl n = node_setup(P, cur, y, phi);
-            const uint32_t p = pidx(P, x, y, z);
-            unroll<0, Q>([&](auto Ic) {
-                constexpr int I = decltype(Ic)::value;
-                float a, b;
-                collide_dir<I>(n, a, b);
-                Store<ST>::template store<I>(B.f_dst, uint32_t(P.PX), p, a, b);
+            const NodeColl n = node_setup(P, m, y, phi);
+            const DevParams PL = fresh_strides(P);
+            const uint32_t p = pidx(PL, x, y, z);
+            collide_node(n, [&](auto Ic, float a, float b) {
+                Store<ST>::template store<decltype(Ic)::value>(B.f_dst, uint32_t(PL.PX), p, a, b);
             });
             if (P.y_mode == MCG_Y_JET) {
-                if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, cur, x, z);
-                if (y == P.NY - 2) outlet_emit(P, B.gB_src, B.gB_dst, B.out_dst, cur, x, z);
+                if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z);
+                if (y == P.NY - 2) outlet_emit(P, B.gB_src, B.gB_dst, B.out_dst, m, x, z);
             }
             if constexpr (STATS) {
-                float dx, dz, r;
-                const int bin = radial_bin(P, x, z, dx, dz, r);
+                float dx, dz, rr;
+                const int bin = radial_bin(P, x, z, dx, dz, rr);
                 if (bin >= 0) {
                     float ur = 0.0f;
-                    if (r > 1.0e-12f) ur = (cur.ux * dx + cur.uz * dz) / r;
-                    const double vy = double(cur.uy), vr = double(ur), ph = double(psi(cur.rr, cur.rb));
-                    double *hb = hist + (size_t(ty) * P.NR + bin) * 8;
+                    if (rr > 1.0e-12f) ur = (m.ux * dx + m.uz * dz) / rr;
+                    const double vy = double(m.uy), vr = double(ur), ph = double(psi(m.rr, m.rb));
+                    double *hb = hist + (size_t(wv) * P.NR + bin) * 8;
                     atomicAdd(hb + 0, vy);
                     atomicAdd(hb + 1, vy * vy);
                     atomicAdd(hb + 2, vr);
@@ -240,13 +310,26 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
                 }
             }
         }
-        cur = nxt;
+        if (lead && active) {
+            float ph;
+            if (split) {
+                nx2 = raw_reduce<ST>(raw);
+                ph = psi(nx2.rr, nx2.rb);
+            } else {
+                bool has_m;
+                ph = gather_pos<ST>(P, B, gx, gy, zl, nx2, has_m);
+            }
+            phs[zl & 3][lr][lc] = ph;
+        }
+        __syncthreads();
+        cur = nx1;
+        nx1 = nx2;
     }
 
     if constexpr (STATS) {
         __syncthreads();
         const size_t nb = size_t(P.NY) * P.NR;
-        for (int k = lid; k < TY * P.NR; k += 64 * TY) {
+        for (int k = lid; k < TY * P.NR; k += NT) {
             const int row = k / P.NR, bin = k % P.NR;
             const int yy = y0 + row;
             const double *hb = hist + size_t(k) * 8;
@@ -259,48 +342,66 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     }
 }
 
-// rows per workgroup of the fused kernel
-constexpr int kFusedTY = 8;
-
-inline FusedGrid fused_grid(const DevParams &P, int zsegs) {
-    FusedGrid G;
-    G.nxt = P.PX / 64;
-    G.nyb = (P.IY + kFusedTY - 1) / kFusedTY;
-    G.nzs = zsegs;
-    G.zlen = (P.IZ + zsegs - 1) / zsegs;
-    G.nzs = (P.IZ + G.zlen - 1) / G.zlen;
-    G.nwg = G.nxt * G.nyb * G.nzs;
-    return G;
+// resident workgroups per CU under the register budget of MCG_FUSED_WAVES_PER_EU
+constexpr int fused_wg_per_cu() {
+    return (4 * MCG_FUSED_WAVES_PER_EU) / kFusedWaves > 0 ? (4 * MCG_FUSED_WAVES_PER_EU) / kFusedWaves : 1;
 }
 
-// z segments: aim at ~3 workgroups per CU over 256 CUs, segments of >= 16 planes
-inline int fused_zsegs(const DevParams &P) {
-    const int tiles = (P.PX / 64) * ((P.IY + kFusedTY - 1) / kFusedTY);
-    int zs = (768 + tiles - 1) / tiles;
-    zs = std::max(1, std::min(zs, P.IZ / 16));
-    return std::max(1, zs);
+// z segments: minimise rounds of resident workgroups x (segment length + 2 fill planes)
+inline FusedGrid fused_grid(const DevParams &P, int num_cu) {
+    FusedGrid best{};
+    double best_cost = 1e300;
+    const int nxt = P.PX / 64, nyb = (P.IY + kFusedTY - 1) / kFusedTY;
+    const int slots = std::max(1, num_cu * fused_wg_per_cu());
+    const int maxseg = std::max(1, std::min(64, P.IZ / 8));
+    for (int zs = 1; zs <= maxseg; ++zs) {
+        FusedGrid G;
+        G.nxt = nxt;
+        G.nyb = nyb;
+        G.zlen = (P.IZ + zs - 1) / zs;
+        G.nzs = (P.IZ + G.zlen - 1) / G.zlen;
+        G.nwg = G.nxt * G.nyb * G.nzs;
+        const int rounds = (G.nwg + slots - 1) / slots;
+        const double cost = double(rounds) * double(G.zlen + 2);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = G;
+        }
+    }
+    return best;
 }
 
-inline bool fused_has_stats(const DevParams &P) { return size_t(kFusedTY) * P.NR * 8 * sizeof(double) <= 65536; }
+inline size_t fused_stats_lds(const DevParams &P) { return size_t(kFusedTY) * P.NR * 8 * sizeof(double); }
+inline size_t fused_static_lds() { return size_t(4) * (kFusedTY + 2) * 66 * sizeof(float); }
+// the histogram fits next to the psi ring in the LDS a workgroup may use
+inline bool fused_has_stats(const DevParams &P, size_t lds_per_block) {
+    return fused_stats_lds(P) + fused_static_lds() <= lds_per_block;
+}
 
 template <int ST>
-inline int launch_fused_st(const DevParams &P, const DevBufs &B, hipStream_t stream) {
-    const FusedGrid G = fused_grid(P, fused_zsegs(P));
-    const dim3 block(64, kFusedTY, 1);
+inline int launch_fused_st(const DevParams &P, const DevBufs &B, hipStream_t stream, int num_cu) {
+    const FusedGrid G = fused_grid(P, num_cu);
+    const dim3 block(64, kFusedWaves, 1);
     if (B.do_stats) {
-        const size_t shm = size_t(kFusedTY) * P.NR * 8 * sizeof(double);
-        hipLaunchKernelGGL((k_fused<kFusedTY, true, ST>), dim3(G.nwg), block, shm, stream, P, B, G);
+        hipLaunchKernelGGL((k_fused<true, ST>), dim3(G.nwg), block, fused_stats_lds(P), stream, P, B, G);
     } else {
-        hipLaunchKernelGGL((k_fused<kFusedTY, false, ST>), dim3(G.nwg), block, 0, stream, P, B, G);
+        hipLaunchKernelGGL((k_fused<false, ST>), dim3(G.nwg), block, 0, stream, P, B, G);
     }
     return int(hipGetLastError());
 }
 
-inline int launch_fused(const DevParams &P, const DevBufs &B, hipStream_t stream) {
+// dynamic LDS above the default limit must be allowed per kernel
+template <int ST>
+inline int fused_allow_lds(size_t bytes) {
+    return int(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fused<true, ST>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+}
+
+inline int launch_fused(const DevParams &P, const DevBufs &B, hipStream_t stream, int num_cu) {
     switch (P.storage) {
-        case MCG_STORE_FP16: return launch_fused_st<MCG_STORE_FP16>(P, B, stream);
-        case MCG_STORE_BF16: return launch_fused_st<MCG_STORE_BF16>(P, B, stream);
-        default: return launch_fused_st<MCG_STORE_FP32>(P, B, stream);
+        case MCG_STORE_FP16: return launch_fused_st<MCG_STORE_FP16>(P, B, stream, num_cu);
+        case MCG_STORE_BF16: return launch_fused_st<MCG_STORE_BF16>(P, B, stream, num_cu);
+        default: return launch_fused_st<MCG_STORE_FP32>(P, B, stream, num_cu);
     }
 }
 

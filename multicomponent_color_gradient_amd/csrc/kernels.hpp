@@ -155,11 +155,8 @@ __global__ __launch_bounds__(256) void k_collide(DevParams P, DevBufs B) {
         return ghost_psi(P, B, xn, yn, zn);
     };
     const NodeColl n = node_setup(P, m, y, phi);
-    unroll<0, Q>([&](auto Ic) {
-        constexpr int I = decltype(Ic)::value;
-        float a, b;
-        collide_dir<I>(n, a, b);
-        Store<ST>::template store<I>(B.f_dst, uint32_t(P.PX), p, a, b);
+    collide_node(n, [&](auto Ic, float a, float b) {
+        Store<ST>::template store<decltype(Ic)::value>(B.f_dst, uint32_t(P.PX), p, a, b);
     });
     if (P.y_mode == MCG_Y_JET) {
         if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z);

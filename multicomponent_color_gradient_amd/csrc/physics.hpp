@@ -70,10 +70,16 @@ __device__ __forceinline__ float fneqr(const Moments &m) {
     return w(I) * (A2 + A3);
 }
 
-// Accumulators of the moments pass, lbm.cuh:82-165 (sum order i = 0..26).
+// Accumulators of the moments pass, lbm.cuh:82-165.  The colour densities are
+// summed in direction order i = 0..26 (psi of a halo position repeats exactly
+// these sums).  The diagonal second moments accumulate g H_aa (H_aa = c_a^2 -
+// cs^2) with one FMA each, as the reference does: its terms of both signs keep
+// the partial sums near Pi_aa instead of near rho/3, which keeps ~3x more of
+// the FP32 precision of Pi^neq than summing g c_a^2 and subtracting cs^2 rho.
+// Off-diagonal moments need only additions (H_ab = +-1 or 0).
 struct MomentAcc {
     float sr = 0.f, sb = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
-    float axx = 0.f, axy = 0.f, ayy = 0.f, ayz = 0.f, azz = 0.f, axz = 0.f;
+    float sxx = 0.f, sxy = 0.f, syy = 0.f, syz = 0.f, szz = 0.f, sxz = 0.f;
 
     template <int I>
     __device__ __forceinline__ void add(float fr, float fb) {
@@ -83,12 +89,12 @@ struct MomentAcc {
         if constexpr (kCX[I] > 0) jx += g; else if constexpr (kCX[I] < 0) jx -= g;
         if constexpr (kCY[I] > 0) jy += g; else if constexpr (kCY[I] < 0) jy -= g;
         if constexpr (kCZ[I] > 0) jz += g; else if constexpr (kCZ[I] < 0) jz -= g;
-        axx += g * Hxx(I);
-        if constexpr (Hxy(I) > 0.0f) axy += g; else if constexpr (Hxy(I) < 0.0f) axy -= g;
-        ayy += g * Hyy(I);
-        if constexpr (Hyz(I) > 0.0f) ayz += g; else if constexpr (Hyz(I) < 0.0f) ayz -= g;
-        azz += g * Hzz(I);
-        if constexpr (Hxz(I) > 0.0f) axz += g; else if constexpr (Hxz(I) < 0.0f) axz -= g;
+        sxx = __builtin_fmaf(g, Hxx(I), sxx);
+        syy = __builtin_fmaf(g, Hyy(I), syy);
+        szz = __builtin_fmaf(g, Hzz(I), szz);
+        if constexpr (Hxy(I) > 0.0f) sxy += g; else if constexpr (Hxy(I) < 0.0f) sxy -= g;
+        if constexpr (Hyz(I) > 0.0f) syz += g; else if constexpr (Hyz(I) < 0.0f) syz -= g;
+        if constexpr (Hxz(I) > 0.0f) sxz += g; else if constexpr (Hxz(I) < 0.0f) sxz -= g;
     }
 
     __device__ __forceinline__ Moments finish() const {
@@ -100,12 +106,12 @@ struct MomentAcc {
         m.ux = jx * inv;
         m.uy = jy * inv;
         m.uz = jz * inv;
-        m.pxx = axx - rho * m.ux * m.ux;
-        m.pxy = axy - rho * m.ux * m.uy;
-        m.pyy = ayy - rho * m.uy * m.uy;
-        m.pyz = ayz - rho * m.uy * m.uz;
-        m.pzz = azz - rho * m.uz * m.uz;
-        m.pxz = axz - rho * m.ux * m.uz;
+        m.pxx = sxx - rho * m.ux * m.ux;
+        m.pxy = sxy - rho * m.ux * m.uy;
+        m.pyy = syy - rho * m.uy * m.uy;
+        m.pyz = syz - rho * m.uy * m.uz;
+        m.pzz = szz - rho * m.uz * m.uz;
+        m.pxz = sxz - rho * m.ux * m.uz;
         return m;
     }
 };
@@ -132,57 +138,82 @@ __device__ __forceinline__ void color_gradient(PhiFn &&phi, float &Fx, float &Fy
     Fz = inv_cs2 * sz;
 }
 
-// Perturbation operator with the cos^2 rule, collisionOperators.cuh:15-24 and
-// forces.cuh:50-75.
-template <int I>
-__device__ __forceinline__ float omega2(float Fx, float Fy, float Fz, float absF, float A) {
-    float c2 = 0.0f;
-    if constexpr (I != 0) {
-        if (!(absF <= 1.0e-4f)) {
-            const float d = cdot<I>(Fx, Fy, Fz);
-            c2 = (d * d) / (absF * absF);
-        }
-    }
-    return (A / 2.0f) * absF * (w(I) * c2 - Bc(I));
-}
-
-// Latva-Kokko recolouring increment, collisionOperators.cuh:28-69.
-// coeff = beta * rr*rb/rho^2 and invF = 1/|F| are per node.
-template <int I>
-__device__ __forceinline__ float recolor(float rho, float coeff, float Fx, float Fy, float Fz, float absF, float invF) {
-    if constexpr (I == 0) {
-        return 0.0f;
-    } else {
-        float c = 0.0f;
-        if (!(absF <= 1.0e-4f)) c = cdot<I>(Fx, Fy, Fz) * invF * invc(I);
-        const float fe = w(I) * rho;
-        return coeff * c * fe;
-    }
-}
-
-// Node-level collision inputs shared by all 27 directions (lbm.cuh:184-250).
+// Collision of one node (lbm.cuh:184-291, forces.cuh:50-105,
+// collisionOperators.cuh:15-69) in even/odd form.  With c -> -c every term of
+// the post-collision population is even (rho, the second-order Hermite terms
+// of feq and fneqr, the perturbation Omega2) or odd (the first- and
+// third-order terms, the recolouring increment), so the pair (i, opp(i))
+// shares one evaluation:
+//   g_{+-} = w_i (E +- O),   f_r = aR g + w_i D,   f_b = aB g - w_i D  (D odd)
+// with, per node,
+//   E = rho (1 - u.u/2cs^2) + rho (c.u)^2/2cs^4 + (1-omega) H2:Pi/2cs^4 + Omega2/w
+//   O = rho (c.u)(1/cs^2 + (c.u)^2/6cs^6 - u.u/2cs^4) + (1-omega) H3:T/2cs^6
+//   Omega2/w = (A/2)|F| ((c.F)^2/|F|^2 - 1)   (B_i = w_i for i != 0; the rest
+//              direction gets (A/2)|F| (10/27)/(8/27))
+//   D = beta rr rb/rho^2 rho (c.F)/(|c||F|)
+// Bulk nodes (4 rr rb/rho^2 <= 1e-4) carry zero Omega2 and D coefficients, so
+// they reduce exactly to f = a g as in the reference's bulk branch, and so do
+// the |F| <= 1e-4 cuts of the cos^2 rule and the recolouring.
 struct NodeColl {
-    Moments m;
-    float rT, aR, aB, oms;
-    bool bulk;
-    float Fx, Fy, Fz, absF, invF, A, coeff;
+    float aR, aB;
+    float ux, uy, uz;
+    float e0, e1, o1, o3;  // feq: E += e0 + e1 (c.u)^2, O += (c.u)(o1 + o3 (c.u)^2)
+    float q2, q3;          // (1-omega)/2cs^4, (1-omega)/2cs^6
+    float tr;              // -cs^2 tr(Pi): the diagonal remainder of H2:Pi
+    float pxx, pyy, pzz, dxy, dyz, dxz;  // Pi diagonal and 2 Pi off-diagonal
+    float txxy, txxz, txyy, txzz, tyyz, tyzz, txyz;  // third-order products of fneqr
+    float Fx, Fy, Fz;
+    float s2, sB;          // Omega2/w = s2 (c.F)^2 - sB
+    float rc;              // D = rc (c.F)/|c|
 };
 
-// Post-collision populations of direction I, lbm.cuh:218-245 (bulk) and
-// :260-291 (interface): g = feq + (1-omega) fneqr [+ Omega2], f_r = aR g [+ D],
-// f_b = aB g [- D].
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// rest direction: r = aR g0, b = aB g0
+__device__ __forceinline__ void collide_rest(const NodeColl &n, float &r, float &b) {
+    float E = fma_(n.q2, n.tr, n.e0);
+    E = fma_(n.sB, 1.25f, E);
+    const float g = w(0) * E;
+    r = n.aR * g;
+    b = n.aB * g;
+}
+
+// direction I and its opposite I+1 (directions 2k-1 and 2k are opposite)
 template <int I>
-__device__ __forceinline__ void collide_dir(const NodeColl &n, float &outr, float &outb) {
-    float g = feq<I>(n.rT, n.m.ux, n.m.uy, n.m.uz) + n.oms * fneqr<I>(n.m);
-    if (n.bulk) {
-        outr = n.aR * g;
-        outb = n.aB * g;
-    } else {
-        g = g + omega2<I>(n.Fx, n.Fy, n.Fz, n.absF, n.A);
-        const float d = recolor<I>(n.rT, n.coeff, n.Fx, n.Fy, n.Fz, n.absF, n.invF);
-        outr = n.aR * g + d;
-        outb = n.aB * g - d;
-    }
+__device__ __forceinline__ void collide_pair(const NodeColl &n, float &rp, float &bp, float &rm, float &bm) {
+    static_assert(I % 2 == 1 && kCX[I] == -kCX[I + 1] && kCY[I] == -kCY[I + 1] && kCZ[I] == -kCZ[I + 1], "pair");
+    const float cu = cdot<I>(n.ux, n.uy, n.uz);
+    const float cu2 = cu * cu;
+    float a2 = n.tr;
+    if constexpr (kCX[I] != 0) a2 += n.pxx;
+    if constexpr (kCY[I] != 0) a2 += n.pyy;
+    if constexpr (kCZ[I] != 0) a2 += n.pzz;
+    if constexpr (Hxy(I) > 0.0f) a2 += n.dxy; else if constexpr (Hxy(I) < 0.0f) a2 -= n.dxy;
+    if constexpr (Hyz(I) > 0.0f) a2 += n.dyz; else if constexpr (Hyz(I) < 0.0f) a2 -= n.dyz;
+    if constexpr (Hxz(I) > 0.0f) a2 += n.dxz; else if constexpr (Hxz(I) < 0.0f) a2 -= n.dxz;
+    float a3 = 0.0f;
+    if constexpr (Hxxy(I) != 0.0f) a3 = fma_(n.txxy, Hxxy(I), a3);
+    if constexpr (Hxxz(I) != 0.0f) a3 = fma_(n.txxz, Hxxz(I), a3);
+    if constexpr (Hxyy(I) != 0.0f) a3 = fma_(n.txyy, Hxyy(I), a3);
+    if constexpr (Hxzz(I) != 0.0f) a3 = fma_(n.txzz, Hxzz(I), a3);
+    if constexpr (Hyyz(I) != 0.0f) a3 = fma_(n.tyyz, Hyyz(I), a3);
+    if constexpr (Hyzz(I) != 0.0f) a3 = fma_(n.tyzz, Hyzz(I), a3);
+    if constexpr (Hxyz(I) != 0.0f) a3 = fma_(n.txyz, Hxyz(I), a3);
+    const float d = cdot<I>(n.Fx, n.Fy, n.Fz);
+    float E = fma_(n.e1, cu2, n.e0);
+    E = fma_(n.q2, a2, E);
+    E = fma_(n.s2, d * d, E) - n.sB;
+    float O = cu * fma_(n.o3, cu2, n.o1);
+    O = fma_(n.q3, a3, O);
+    float D = n.rc * d;
+    if constexpr (csq(I) > 1) D = D * invc(I);
+    const float gp = w(I) * (E + O);
+    const float gm = w(I) * (E - O);
+    const float Dw = w(I) * D;
+    rp = fma_(n.aR, gp, Dw);
+    bp = fma_(n.aB, gp, -Dw);
+    rm = fma_(n.aR, gm, -Dw);
+    bm = fma_(n.aB, gm, Dw);
 }
 
 // Regularised boundary population (inlet jet/boundary_conditions.cuh:128-147,
