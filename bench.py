@@ -1,22 +1,25 @@
 """Throughput benchmark of the D3Q27 colour-gradient step on one MI355X.
 
 Workload (default): the round jet on 256x512x256 nodes (ghost layers
-included), inlet disk R=16 lu, U=0.05, tau_r/tau_b = 0.55/0.6 (the reference
-relaxes with tau_b everywhere), A=1e-3, beta=0.7, FP32 populations, radial
-statistics sampled every 10 steps.  Synthetic initial state: the reference's
-jet initialisation (jet/jetIn.cuh).  A step is one full time step (moments,
-colour gradient, collision, perturbation, recolouring, streaming, inlet,
-outlet, statistics when due) over the whole grid, state resident in HBM.
+included), inlet disk R=16 lu with the tanh-edged top-hat U_in=0.05 (w=2) and
+the seeded uniform transverse perturbation of 1% of U_in (SplitMix64, seed
+20261203), ambient fluid at rest, tau_r/tau_b = 0.55/0.6 (the collision relaxes
+with tau_b, the reference's single rate), A=1e-3, beta=0.7, FP32 populations,
+radial statistics of u_y, u_r and phi sampled every 10 steps from the first
+step (so their cost is inside the timed window).  A step is one full time step
+(moments, colour gradient, collision, perturbation, recolouring, streaming,
+inlet, outlet, statistics when due) over the whole grid, state resident in HBM.
 
 MLUPS follows utilities/countmlups.cuh:42-48 (steps * NX*NY*NZ / t, ghost
 layers counted); the interior-only figure is reported too.  The roofline
 uses 432 algorithmic bytes per interior lattice update (27 populations x 2
-colours read + written, FP32) against 8 TB/s.
+colours read + written, FP32; 216 B with 16-bit storage) against 8 TB/s.
 
 The CPU baseline runs the FP64 OpenMP oracle (oracle/, a restatement of the
 reference step) on the same grid for a bounded number of steps.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload jet256|jet128|bubble64]
+                       [--storage fp32|fp16|bf16] [--kernel fused|twopass]
 """
 from __future__ import annotations
 
@@ -37,17 +40,14 @@ def make_case(name: str):
     import multicomponent_color_gradient_amd as m
 
     if name == "jet256":
-        c = m.jet_case(256, 512, 256, jet_radius=16.0, jet_velocity=0.05)
+        c = m.perturbed_jet_case(256, 512, 256, jet_radius=16.0)
     elif name == "jet128":
-        c = m.jet_case(128, 256, 128, jet_radius=8.0, jet_velocity=0.05)
+        c = m.perturbed_jet_case(128, 256, 128, jet_radius=8.0)
     elif name == "bubble64":
         c = m.periodic_bubble_case(64, radius=12.0, tau=0.8, A=1e-3, beta=0.7)
         return c, "bubble_tanh"
     else:
         raise SystemExit(f"unknown workload {name}")
-    c.taur, c.taub = 0.55, 0.6
-    c.A_override = 1e-3
-    c.beta_recolor = 0.7
     c.NSTATS_START_STEP = 0
     c.NSTATS_SAMPLE = 10
     return c, "jet"
@@ -171,8 +171,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.storage == "fp32" else f"{args.storage}-storage/f32-math",
-        "data": "synthetic (reference jet initialisation, sharp inlet)",
-        "config": {"workload": f"round jet {case.NX}x{case.NY}x{case.NZ}, R={case.jet_radius:g}, U={case.jet_velocity:g}"
+        "data": ("synthetic (jet initialisation at rest, tanh inlet with seeded 1% transverse perturbation)"
+                 if init == "jet" else "synthetic (static tanh bubble)"),
+        "config": {"workload": f"round jet {case.NX}x{case.NY}x{case.NZ}, R={case.jet_radius:g}, U={case.jet_velocity:g}, "
+                               f"tanh inlet + 1% perturbation, tau {case.taur:g}/{case.taub:g}, A={case.A_override:g}, "
+                               f"beta={case.beta_recolor:g}"
                                if init == "jet" else f"periodic tanh bubble {case.NX}^3",
                    "NX": case.NX, "NY": case.NY, "NZ": case.NZ, "storage": args.storage, "kernel": args.kernel,
                    "stats_every": case.NSTATS_SAMPLE, "mlups_interior": round(mlups_int, 2),

@@ -11,7 +11,8 @@ derivations:
 """
 from __future__ import annotations
 
-from ._abi import (MCG_INLET_SHARP, MCG_VISC_INTERFACE, MCG_VISC_SINGLE, MCG_Y_JET, MCG_Y_PERIODIC, McgCase)
+from ._abi import (MCG_INLET_SHARP, MCG_INLET_TANH, MCG_VISC_INTERFACE, MCG_VISC_SINGLE, MCG_Y_JET, MCG_Y_PERIODIC,
+                   McgCase)
 
 CS2 = 1.0 / 3.0
 
@@ -60,6 +61,30 @@ def jet_case(NX: int, NY: int, NZ: int, jet_radius: float, jet_velocity: float =
     nu = (2.0 * c.jet_radius * c.jet_velocity) / c.Re
     c.taur = c.taub = 0.5 + nu / CS2
     c.sigma = (c.rhob0 * c.jet_velocity * c.jet_velocity * 2.0 * c.jet_radius) / c.We
+    c.beta_recolor = float(beta)
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+PERTURBATION_SEED = 20261203
+
+
+def perturbed_jet_case(NX: int, NY: int, NZ: int, jet_radius: float, jet_velocity: float = 0.05,
+                       perturbation: float = 0.01, seed: int = PERTURBATION_SEED, width: float = 2.0,
+                       taur: float = 0.55, taub: float = 0.6, A: float = 1e-3, beta: float = 0.7,
+                       **overrides) -> McgCase:
+    """Round jet with the tanh-edged top-hat inlet u_y = U (1 - tanh((r - R)/width))/2 and a
+    seeded uniform transverse perturbation of amplitude ``perturbation`` x U (MCG_INLET_TANH, an
+    extension beyond the reference's sharp inlet).  The collision relaxes with taub everywhere
+    (the reference's single rate); taur is kept for the interface-viscosity option."""
+    c = jet_case(NX, NY, NZ, jet_radius, jet_velocity)
+    c.inlet_profile = MCG_INLET_TANH
+    c.interface_width = float(width)
+    c.perturbation_amp = float(perturbation)
+    c.perturbation_seed = int(seed)
+    c.taur, c.taub = float(taur), float(taub)
+    c.A_override = float(A)
     c.beta_recolor = float(beta)
     for k, v in overrides.items():
         setattr(c, k, v)

@@ -113,6 +113,7 @@ struct mcg_solver {
         B.stats = stats;
         B.count = count;
         B.do_stats = do_stats ? 1 : 0;
+        B.step = (unsigned long long)(step);
         return B;
     }
 
@@ -166,6 +167,10 @@ void fill_params(const mcg_case &c, int storage, DevParams &P) {
     P.rhob0 = float(c.rhob0);
     P.NR = c.NX / 2;
     P.storage = storage;
+    P.inlet_profile = c.inlet_profile;
+    P.inlet_w = float(c.interface_width);
+    P.pert_amp = float(c.perturbation_amp);
+    P.pert_seed = uint32_t(c.perturbation_seed);
 }
 
 int check_solver(const mcg_solver *s) {
@@ -356,7 +361,11 @@ int mcg_case_validate(const mcg_case *c) {
     if (c->y_mode != MCG_Y_JET && c->y_mode != MCG_Y_PERIODIC) return fail(MCG_E_INVALID, "unknown y_mode");
     if (c->viscosity_mode != MCG_VISC_SINGLE && c->viscosity_mode != MCG_VISC_INTERFACE)
         return fail(MCG_E_INVALID, "unknown viscosity_mode");
-    if (c->inlet_profile != MCG_INLET_SHARP) return fail(MCG_E_UNSUPPORTED, "only the sharp inlet profile is implemented");
+    if (c->inlet_profile != MCG_INLET_SHARP && c->inlet_profile != MCG_INLET_TANH)
+        return fail(MCG_E_INVALID, "unknown inlet_profile");
+    if (c->inlet_profile == MCG_INLET_TANH && !(c->interface_width > 0))
+        return fail(MCG_E_INVALID, "interface_width must be > 0 for the tanh inlet");
+    if (!(c->perturbation_amp >= 0.0)) return fail(MCG_E_INVALID, "perturbation_amp must be >= 0");
     if (!(c->taub > 0.5) || !(c->taur > 0.5)) return fail(MCG_E_INVALID, "relaxation times must exceed 0.5");
     if (c->sponge_cells < 2 && c->sponge_K != 0.0) return fail(MCG_E_INVALID, "sponge_cells must be >= 2");
     if (c->rhor0 <= 0 || c->rhob0 <= 0) return fail(MCG_E_INVALID, "reference densities must be > 0");

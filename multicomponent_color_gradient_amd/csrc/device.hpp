@@ -54,6 +54,10 @@ struct DevParams {
     float rhor0, rhob0;
     int NR;           // radial bins, NX/2 (constants.cuh:20)
     int storage;      // MCG_STORE_*
+    int inlet_profile;   // MCG_INLET_*
+    float inlet_w;       // tanh width of the MCG_INLET_TANH profile
+    float pert_amp;      // transverse perturbation amplitude, fraction of jet_U
+    uint32_t pert_seed;  // seed of the perturbation
 };
 
 struct DevBufs {
@@ -71,6 +75,7 @@ struct DevBufs {
     double *stats;         // 7 planes of NY*NR (uy, uy2, ur, ur2, uruy, phi, phi2)
     unsigned long long *count;  // NY*NR
     int do_stats;
+    unsigned long long step;    // index of the step being computed (0 for the first)
 };
 
 // population row-block index of interior node (x,y,z): the 54 (FP32) or 27
@@ -402,9 +407,46 @@ __device__ __forceinline__ void collide_node(const NodeColl &n, PutFn &&put) {
     });
 }
 
+// SplitMix64 (Steele, Lea and Flood 2014, the finaliser constants of Vigna's
+// reference implementation): a counter-based generator, so the CPU and the GPU
+// draw the same numbers for the same (seed, step, node).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// uniform number in [0,1) (24 bits) number c of inlet column (x,z) at step n
+__device__ __forceinline__ float inlet_xi(uint32_t seed, uint64_t step, int x, int z, int NX, int c) {
+    const uint64_t key = splitmix64(splitmix64(splitmix64(uint64_t(seed)) ^ step) ^ (uint64_t(z) * uint64_t(NX) + uint64_t(x)));
+    return float(splitmix64(key + uint64_t(c)) >> 40) * (1.0f / 16777216.0f);
+}
+
+// Inlet velocity of jet ghost node (x,0,z).  Sharp profile: (0, U, 0)
+// (jet/boundary_conditions.cuh:62-75).  MCG_INLET_TANH (an extension beyond
+// the reference): u_y = U (1 - tanh((r - R)/w))/2 on the jet disk, and
+// transverse u_x, u_z = amp U (2 xi - 1) with xi drawn per step and column.
+__device__ __forceinline__ void inlet_velocity(const DevParams &P, uint64_t step, int x, int z, float &ux, float &uy,
+                                               float &uz) {
+    ux = 0.0f;
+    uy = P.jet_U;
+    uz = 0.0f;
+    if (P.inlet_profile == MCG_INLET_TANH) {
+        const float dx = float(x) - P.jet_x0;
+        const float dz = float(z) - P.jet_z0;
+        const float r = sqrtf(dx * dx + dz * dz);
+        uy = P.jet_U * (0.5f * (1.0f - tanhf((r - P.jet_R) / P.inlet_w)));
+        const float a = P.pert_amp * P.jet_U;
+        ux = a * (2.0f * inlet_xi(P.pert_seed, step, x, z, P.NX, 1) - 1.0f);
+        uz = a * (2.0f * inlet_xi(P.pert_seed, step, x, z, P.NX, 2) - 1.0f);
+    }
+}
+
 // Inlet populations sent by ghost node (x,0,z) of a jet column, computed from
 // the moments m of node (x,1,z): jet/boundary_conditions.cuh:97-148.
-__device__ __forceinline__ void inlet_emit(const DevParams &P, float *__restrict__ in_dst, const Moments &m, int x, int z) {
+__device__ __forceinline__ void inlet_emit(const DevParams &P, float *__restrict__ in_dst, const Moments &m, int x, int z,
+                                           uint64_t step) {
     const float rrB = (1.0f - 1.0f) * P.rhor0;  // jet node: is_jet = 1
     const float rbB = 1.0f * P.rhob0;
     const float rT = rrB + rbB;
@@ -413,12 +455,14 @@ __device__ __forceinline__ void inlet_emit(const DevParams &P, float *__restrict
     float om = P.omega_b;
     if (P.visc_mode == MCG_VISC_INTERFACE) om = 1.0f / tau_interface(P, rrB, rbB);
     const float oms = 1.0f - om;
+    float ex, ey, ez;
+    inlet_velocity(P, step, x, z, ex, ey, ez);
     const size_t nxz = size_t(P.NX) * P.NZ;
     const size_t o = size_t(z) * P.NX + x;
     unroll<0, 9>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         constexpr int I = kUp[k];
-        const float g = boundary_pop<I>(rT, 0.0f, P.jet_U, 0.0f, oms, m);
+        const float g = boundary_pop<I>(rT, ex, ey, ez, oms, m);
         in_dst[(2 * k) * nxz + o] = aR * g;
         in_dst[(2 * k + 1) * nxz + o] = aB * g;
     });

@@ -288,7 +288,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
                 Store<ST>::template store<decltype(Ic)::value>(B.f_dst, uint32_t(PL.PX), p, a, b);
             });
             if (P.y_mode == MCG_Y_JET) {
-                if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z);
+                if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z, B.step);
                 if (y == P.NY - 2) outlet_emit(P, B.gB_src, B.gB_dst, B.out_dst, m, x, z);
             }
             if constexpr (STATS) {

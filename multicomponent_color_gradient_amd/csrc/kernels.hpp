@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_collide(DevParams P, DevBufs B) {
         Store<ST>::template store<decltype(Ic)::value>(B.f_dst, uint32_t(P.PX), p, a, b);
     });
     if (P.y_mode == MCG_Y_JET) {
-        if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z);
+        if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z, B.step);
         if (y == P.NY - 2) outlet_emit(P, B.gB_src, B.gB_dst, B.out_dst, m, x, z);
     }
 }

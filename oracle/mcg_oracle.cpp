@@ -430,6 +430,37 @@ struct Oracle final : OracleBase {
                 }
     }
 
+    // SplitMix64 (Steele, Lea and Flood 2014; Vigna's reference constants), the
+    // counter-based generator of the MCG_INLET_TANH perturbation
+    static uint64_t splitmix64(uint64_t x) {
+        x += 0x9E3779B97F4A7C15ull;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        return x ^ (x >> 31);
+    }
+    // uniform in [0,1) with 24 bits: number k of column (x,z) at step n
+    double inlet_xi(int64_t n, int x, int z, int k) const {
+        const uint64_t key = splitmix64(splitmix64(splitmix64(uint64_t(uint32_t(c.perturbation_seed))) ^ uint64_t(n)) ^
+                                        (uint64_t(z) * uint64_t(NX) + uint64_t(x)));
+        return double(splitmix64(key + uint64_t(k)) >> 40) / 16777216.0;
+    }
+    // Inlet velocity: (0, U, 0) for the reference's sharp profile
+    // (jet/boundary_conditions.cuh:62-75); for MCG_INLET_TANH (an extension)
+    // u_y = U (1 - tanh((r - R)/w))/2 and u_x, u_z = amp U (2 xi - 1)
+    void inlet_velocity(int x, int z, T &ex, T &ey, T &ez) const {
+        ex = T(0);
+        ey = T(c.jet_velocity);
+        ez = T(0);
+        if (c.inlet_profile == MCG_INLET_TANH) {
+            const T dx = T(x) - T(c.jet_x0), dz = T(z) - T(c.jet_z0);
+            const T r = std::sqrt(dx * dx + dz * dz);
+            ey = T(c.jet_velocity) * (T(0.5) * (T(1) - std::tanh((r - T(c.jet_radius)) / T(c.interface_width))));
+            const T a = T(c.perturbation_amp) * T(c.jet_velocity);
+            ex = a * (T(2) * T(inlet_xi(step_, x, z, 1)) - T(1));
+            ez = a * (T(2) * T(inlet_xi(step_, x, z, 2)) - T(1));
+        }
+    }
+
     // Inlet, jet/boundary_conditions.cuh:17-85 (inner (x,z) only, kernels.cu:98-114)
     void inlet() {
         for (int z = 1; z <= NZ - 2; ++z)
@@ -446,10 +477,12 @@ struct Oracle final : OracleBase {
                 T om = T(1) / taub_;  // omegab, constants.cuh:67
                 if (c.viscosity_mode == MCG_VISC_INTERFACE) om = T(1) / tau_node(rhor[idB], rhob[idB]);
                 const T oms = T(1) - om;
+                T ex, ey, ez;
+                inlet_velocity(x, z, ex, ey, ez);
                 for (int i = 0; i < Q; ++i) {
                     if (CY[i] != 1) continue;
                     const size_t t = idx(x + CX[i], 1, z + CZ[i]);  // offset_xz, no wrap
-                    const T g = feq<T>(L, i, rT, T(0), T(c.jet_velocity), T(0)) + oms * fneqr<T>(L, i, P, uxf, uyf, uzf);
+                    const T g = feq<T>(L, i, rT, ex, ey, ez) + oms * fneqr<T>(L, i, P, uxf, uyf, uzf);
                     put(fr, cnt_r, fidx(t, i), aR * g);
                     put(fb, cnt_b, fidx(t, i), aB * g);
                 }

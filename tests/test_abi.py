@@ -57,7 +57,8 @@ def test_case_struct_layout_and_reference_defaults():
     ("NX", 4096, _abi.MCG_E_INVALID),
     ("y_mode", 7, _abi.MCG_E_INVALID),
     ("taub", 0.5, _abi.MCG_E_INVALID),
-    ("inlet_profile", 5, _abi.MCG_E_UNSUPPORTED),
+    ("inlet_profile", 5, _abi.MCG_E_INVALID),
+    ("perturbation_amp", -0.1, _abi.MCG_E_INVALID),
 ])
 def test_case_validation_errors(field, value, code):
     lib = m.load_library()

@@ -84,6 +84,36 @@ def test_jet_error_is_fp32_rounding(mcg, steps):
         assert eg <= 3 * eq + 1e-9, f"{k}: gpu {eg:.3e} vs fp32 restatement {eq:.3e}"
 
 
+@pytest.mark.parametrize("steps,scale", [(10, 1.0), (100, 10.0)])
+def test_perturbed_jet_matches_oracle(mcg, steps, scale):
+    """tanh-edged inlet with the seeded transverse perturbation (both sides draw
+    SplitMix64 numbers independently)."""
+    case = mcg.perturbed_jet_case(26, 22, 24, jet_radius=5.0)
+    s, o = run_pair(mcg, case, steps)
+    assert_fields(s, o, scale, f"perturbed jet {steps} steps")
+    assert_pops(s, o, 2e-6 * scale, f"perturbed jet {steps} steps")
+
+
+def test_static_bubble_laplace(mcg):
+    """The static-bubble configuration: 64^3 periodic, R=12, tau 0.8, A=1e-3,
+    beta 0.7, tanh interface w=2, 2000 steps.  dp = 2 sigma/R with the realised
+    tension sigma = (2/9) A tau (the oracle's test_laplace_law) within 5%, and
+    the state after 200 steps matches the oracle."""
+    case = mcg.periodic_bubble_case(64, radius=12.0, tau=0.8, A=1e-3, beta=0.7, width=2.0)
+    s, o = run_pair(mcg, case, 200, init="bubble_tanh")
+    assert_fields(s, o, 10.0, "static bubble 200 steps")
+    s.step(1800)
+    f = s.fields()
+    rho = f["rhor"].astype(np.float64) + f["rhob"]
+    N = 64
+    z, y, x = np.meshgrid(*[np.arange(N)] * 3, indexing="ij")
+    r = np.sqrt((x - case.bubble_x0) ** 2 + (y - case.bubble_y0) ** 2 + (z - case.bubble_z0) ** 2)
+    interior = (x > 0) & (x < N - 1) & (y > 0) & (y < N - 1) & (z > 0) & (z < N - 1)
+    dp = (rho[r < 8].mean() - rho[(r > 17) & interior].mean()) / 3.0
+    sigma = 2.0 / 9.0 * 1e-3 * 0.8
+    assert abs(dp * 12.0 / (2 * sigma) - 1) < 0.05, dp * 12.0 / (2 * sigma)
+
+
 def test_jet_odd_grid_matches_oracle(mcg):
     # odd NX: integer jet centre, rows not a multiple of 64, radial bins hit r=0
     case = mcg.jet_case(37, 23, 29, jet_radius=5.0, jet_velocity=0.05, Re=500.0)
@@ -185,6 +215,27 @@ def test_radial_statistics_kernels(mcg, kernel):
     assert np.all(np.abs(g["sum_uy"] - r["sum_uy"]) <= 1e-6 * r["count"] + 1e-12)
 
 
+def test_statistics_histogram_above_64k_lds(mcg):
+    """NX=256 gives 128 radial bins: the fused kernel's LDS histogram (9 rows x
+    128 bins x 64 B) exceeds 64 KiB; its statistics must equal the two-pass
+    path's (separate moments + k_radial) and the populations stay bitwise equal."""
+    case = mcg.jet_case(256, 14, 12, jet_radius=4.0, Re=500.0)
+    case.NSTATS_START_STEP = 0
+    case.NSTATS_SAMPLE = 2
+    a = mcg.Solver(case, kernel="fused")
+    b = mcg.Solver(case, kernel="twopass")
+    a.init_jet()
+    b.init_jet()
+    a.step(9)
+    b.step(9)
+    ra, rb = a.radial(), b.radial()
+    assert np.array_equal(ra["count"], rb["count"]) and int(ra["count"].sum()) > 0
+    for k in ("sum_uy", "sum_uy2", "sum_ur", "sum_ur2", "sum_uruy", "sum_phi", "sum_phi2"):
+        assert np.allclose(ra[k], rb[k], rtol=1e-12, atol=1e-15), k
+    pa, pb = a.populations(), b.populations()
+    assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1])
+
+
 def test_tke_and_global_sums(mcg):
     case = mcg.jet_case(24, 20, 24, jet_radius=4.0, Re=500.0)
     s, o = run_pair(mcg, case, 15)
@@ -215,8 +266,12 @@ def test_fused_equals_twopass_bitwise(mcg):
         assert np.array_equal(ga[k], gb[k]), k
 
 
-def test_checkpoint_roundtrip_bitwise(mcg):
-    case = mcg.jet_case(24, 22, 24, jet_radius=4.0, Re=500.0)
+@pytest.mark.parametrize("inlet", ["sharp", "tanh"])
+def test_checkpoint_roundtrip_bitwise(mcg, inlet):
+    if inlet == "sharp":
+        case = mcg.jet_case(24, 22, 24, jet_radius=4.0, Re=500.0)
+    else:  # the perturbation draws depend on the restored step counter
+        case = mcg.perturbed_jet_case(24, 22, 24, jet_radius=4.0)
     case.NSTATS_START_STEP = 2
     a = mcg.Solver(case)
     a.init_jet()

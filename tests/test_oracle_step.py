@@ -3,8 +3,9 @@
 Each test checks a property that the reference's algorithm fixes
 independently of how the oracle is coded: fixed points, conservation laws,
 the write set of the in-place push (quirks Q1/Q2), hand-computed boundary
-values, the lattice viscosity nu = cs^2 (tau - 1/2), exactness of the colour
-gradient on linear fields, and brute-force recomputation of the statistics.
+values, the lattice viscosity nu = cs^2 (tau - 1/2), Laplace's law for a
+static bubble, exactness of the colour gradient on linear fields, and
+brute-force recomputation of the statistics.
 """
 import numpy as np
 import pytest
@@ -164,6 +165,96 @@ def test_inlet_closed_form_after_one_step():
     f = o.fields()
     assert f["rhob"][x0, 0, x0] == 1 and f["rhor"][x0, 0, x0] == 0
     assert f["rhor"][1, 0, 1] == 1 and f["rhob"][1, 0, 1] == 0
+
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(x):
+    """SplitMix64 output for state x (Steele, Lea and Flood 2014; Vigna's constants)."""
+    x = (x + 0x9E3779B97F4A7C15) & MASK64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & MASK64
+    return x ^ (x >> 31)
+
+
+def test_splitmix64_known_answers():
+    # the first outputs of SplitMix64 seeded with 0 (states gamma*k)
+    g = 0x9E3779B97F4A7C15
+    got = [splitmix64((k * g) & MASK64) for k in range(4)]
+    assert got == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC]
+
+
+def inlet_xi(seed, step, x, z, NX, k):
+    key = splitmix64(splitmix64(splitmix64(seed & 0xFFFFFFFF) ^ step) ^ (z * NX + x))
+    return (splitmix64((key + k) & MASK64) >> 40) / 16777216.0
+
+
+def test_tanh_inlet_closed_form_after_one_step():
+    # first step from rest: Pi(y=1) = 0, so every jet column (x,z) sends
+    # f_b = feq_i(1, u_in(x,z)) with u_y = U (1 - tanh((r-R)/w))/2 and
+    # u_x, u_z = amp U (2 xi - 1), xi from SplitMix64 of (seed, step 0, x, z)
+    U, amp, seed, w = 0.05, 0.01, 777, 2.0
+    case = m.perturbed_jet_case(18, 10, 18, jet_radius=3.5, jet_velocity=U, perturbation=amp, seed=seed, width=w)
+    o = Oracle(case)
+    o.init_jet()
+    o.step(1)
+    fr, fb = o.populations()
+    L, c = c_vectors()
+    checked = 0
+    for z in range(1, case.NZ - 1):
+        for x in range(1, case.NX - 1):
+            r = np.hypot(x - case.jet_x0, z - case.jet_z0)
+            if r > case.jet_radius:
+                continue
+            uy = U * 0.5 * (1 - np.tanh((r - case.jet_radius) / w))
+            ux = amp * U * (2 * inlet_xi(seed, 0, x, z, case.NX, 1) - 1)
+            uz = amp * U * (2 * inlet_xi(seed, 0, x, z, case.NX, 2) - 1)
+            for i in np.where(c[:, 1] == 1)[0]:
+                xt, zt = x + c[i, 0], z + c[i, 2]
+                if not (1 <= xt <= case.NX - 2 and 1 <= zt <= case.NZ - 2):
+                    continue
+                assert abs(fb[i, zt, 1, xt] - orc.feq(i, 1.0, ux, uy, uz)) < 1e-15
+                assert fr[i, zt, 1, xt] == 0
+                checked += 1
+    assert checked > 200
+    # the perturbation is bounded by amp U and differs between columns
+    assert 0 < abs(inlet_xi(seed, 0, 8, 8, 18, 1) - inlet_xi(seed, 0, 9, 8, 18, 1)) < 1
+
+
+def laplace_dp(N, R, steps, tau=0.8, A=1e-3):
+    case = m.periodic_bubble_case(N, radius=R, tau=tau, A=A, beta=0.7, width=2.0)
+    o = Oracle(case)
+    o.init_bubble_tanh()
+    o.step(steps)
+    f = o.fields()
+    rho = f["rhor"] + f["rhob"]
+    z, y, x = np.meshgrid(*[np.arange(N)] * 3, indexing="ij")
+    r = np.sqrt((x - case.bubble_x0) ** 2 + (y - case.bubble_y0) ** 2 + (z - case.bubble_z0) ** 2)
+    interior = (x > 0) & (x < N - 1) & (y > 0) & (y < N - 1) & (z > 0) & (z < N - 1)
+    return CS2 * (rho[r < R - 4].mean() - rho[(r > R + 5) & interior].mean())
+
+
+def test_laplace_law():
+    """Static bubble: dp = 2 sigma / R with the realised tension sigma = (2/9) A tau.
+
+    dp R is independent of R, and dp is linear in A and in tau; the absolute
+    value is half of 2 sigma / R for the reference's calibration
+    sigma = 4 cs^4 tau A (constants.cuh:76 with forces.cuh:102-103) -- quirk
+    Q10: the realised tension is (2/9) A tau, the 3-D value of Liu, Valocchi and
+    Kang (2012) for this perturbation operator."""
+    tau, A = 0.8, 1e-3
+    dp8 = laplace_dp(32, 8.0, 1500, tau, A)
+    dp6 = laplace_dp(32, 6.0, 1500, tau, A)
+    dpA = laplace_dp(32, 8.0, 1500, tau, 2 * A)
+    dpt = laplace_dp(32, 8.0, 1500, 1.0, A)
+    sigma = 2.0 / 9.0 * A * tau
+    # the interface is 2 lu wide, so the smaller bubble deviates more (3% at R=8, 6% at R=6)
+    assert abs(dp8 * 8.0 / (2 * sigma) - 1) < 0.05
+    assert abs(dp6 * 6.0 / (2 * sigma) - 1) < 0.08
+    assert abs((dp8 * 8.0) / (dp6 * 6.0) - 1) < 0.05
+    assert abs(dpA / dp8 - 2.0) < 0.01
+    assert abs(dpt / dp8 - 1.25) < 0.01
 
 
 def outlet_case():
