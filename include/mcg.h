@@ -211,6 +211,42 @@ int mcg_checkpoint_size(const mcg_solver *s, uint64_t *nbytes);
 int mcg_checkpoint_save(mcg_solver *s, void *host_buf, uint64_t nbytes);
 int mcg_checkpoint_load(mcg_solver *s, const void *host_buf, uint64_t nbytes);
 
+/* ---- adjoint (extension: the reference has no adjoint) ---------------- */
+/* Discrete adjoint of the step for the periodic domain (y_mode periodic,
+ * wrap_gradient 1), single relaxation rate, FP32 populations.  The state is
+ * the populations in the reference layout (what mcg_copy_populations returns:
+ * f_i(x) after streaming); the parameters are A (the perturbation amplitude,
+ * A_override or sigma/(4 cs^4 taub)) and beta.  Bulk/interface switches and
+ * the |F| <= 1e-4 cut are held at the forward state. */
+
+/* Run nsteps forward steps from the current state and keep a copy of the
+ * populations every checkpoint_every steps (device memory: ceil(nsteps/K) +
+ * K + 4 population sets during mcg_adjoint_backward). */
+int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every);
+
+/* Given lam = dJ/df at the final state (reference layout, 27 x NX*NY*NZ per
+ * colour; ghost slots ignored), return dJ/df at the recorded initial state
+ * (ghost slots 0) and dparams = {dJ/dA, dJ/dbeta} summed in FP64.  Each
+ * segment is recomputed from its checkpoint; statistics are not touched and
+ * the final state is restored. */
+int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb, float *out_lam_fr, float *out_lam_fb,
+                         double *dparams);
+
+/* Device milliseconds of the last mcg_adjoint_backward sweep (segment
+ * recomputation and adjoint kernels, HIP events on the solver's stream;
+ * excludes the host transfers of the seed and the result). */
+int mcg_adjoint_last_ms(const mcg_solver *s, double *ms);
+
+/* Free the adjoint's checkpoints and buffers. */
+int mcg_adjoint_release(mcg_solver *s);
+
+/* The same forward and adjoint on the CPU in FP64 (host code, no GPU):
+ * nsteps steps from (fr0, fb0), final state into (out_fr, out_fb), and, when
+ * lam_fr/lam_fb are given, the adjoint at the initial state and dparams. */
+int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *lam_fr,
+                    const double *lam_fb, double *out_fr, double *out_fb, double *out_lam_fr, double *out_lam_fb,
+                    double *dparams);
+
 /* ---- measurement ------------------------------------------------------ */
 
 /* Run nsteps steps bracketed by HIP events on the solver's stream and return

@@ -134,6 +134,11 @@ SIGNATURES = [
     ("mcg_checkpoint_size", c_int, [_PSOLVER, POINTER(c_uint64)]),
     ("mcg_checkpoint_save", c_int, [_PSOLVER, c_void_p, c_uint64]),
     ("mcg_checkpoint_load", c_int, [_PSOLVER, c_void_p, c_uint64]),
+    ("mcg_adjoint_record", c_int, [_PSOLVER, c_int, c_int]),
+    ("mcg_adjoint_backward", c_int, [_PSOLVER, _F, _F, _F, _F, _D]),
+    ("mcg_adjoint_last_ms", c_int, [_PSOLVER, _D]),
+    ("mcg_adjoint_release", c_int, [_PSOLVER]),
+    ("mcg_cpu_adjoint", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     ("mcg_time_steps", c_int, [_PSOLVER, c_int, _D]),
     ("mcg_device_bytes", c_int, [_PSOLVER, POINTER(c_uint64)]),
     ("mcg_output_dirs", c_int, [POINTER(McgCase), c_char_p, c_char_p, c_char_p, c_char_p, c_size_t]),

@@ -1,0 +1,101 @@
+"""Adjoint of the colour-gradient step: FP64 CPU reference, and a
+``torch.autograd.Function`` over the GPU adjoint of the C ABI.
+
+The state is the populations in the reference layout, shape (27, NZ, NY, NX)
+per colour (f_i(x) after streaming, ghost slots unused); the parameters are
+A (the perturbation amplitude) and beta (the recolouring parameter).  The
+domain is periodic (``periodic_bubble_case``), the relaxation rate single,
+and the storage FP32.  Gradients with respect to A and beta are accumulated
+in FP64.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._abi import McgCase, as_ptr, check, load_library
+from .solver import Solver
+
+
+def cpu_adjoint(case: McgCase, fr0: np.ndarray, fb0: np.ndarray, nsteps: int, lam_fr: np.ndarray | None = None,
+                lam_fb: np.ndarray | None = None) -> dict:
+    """FP64 forward of nsteps steps and, if lam_* is given, the adjoint: returns
+    {"fr", "fb"} (final state) and {"lam_fr", "lam_fb", "dA", "dbeta"}."""
+    lib = load_library()
+    shape = (27, case.NZ, case.NY, case.NX)
+    d = ctypes.c_double
+    fr0 = np.ascontiguousarray(fr0, np.float64).reshape(shape)
+    fb0 = np.ascontiguousarray(fb0, np.float64).reshape(shape)
+    out = {"fr": np.empty(shape), "fb": np.empty(shape)}
+    dp = np.zeros(2)
+    if lam_fr is not None:
+        lam_fr = np.ascontiguousarray(lam_fr, np.float64).reshape(shape)
+        lam_fb = np.ascontiguousarray(lam_fb, np.float64).reshape(shape)
+        out["lam_fr"], out["lam_fb"] = np.empty(shape), np.empty(shape)
+        args = (as_ptr(lam_fr, d), as_ptr(lam_fb, d))
+        outs = (as_ptr(out["lam_fr"], d), as_ptr(out["lam_fb"], d), as_ptr(dp, d))
+    else:
+        args = (None, None)
+        outs = (None, None, None)
+    c = case.copy()
+    check(lib.mcg_cpu_adjoint(ctypes.byref(c), int(nsteps), as_ptr(fr0, d), as_ptr(fb0, d), *args,
+                              as_ptr(out["fr"], d), as_ptr(out["fb"], d), *outs))
+    if lam_fr is not None:
+        out["dA"], out["dbeta"] = float(dp[0]), float(dp[1])
+    return out
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def make_steps_function():
+    """``LBMSteps.apply(fr0, fb0, A, beta, case, nsteps, checkpoint_every)`` -> (fr, fb):
+    nsteps GPU steps from (fr0, fb0) with the case's A_override / beta_recolor
+    replaced by the tensors A and beta; backward runs the GPU adjoint with
+    checkpointed recomputation."""
+    torch = _torch()
+
+    class LBMSteps(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, fr0, fb0, A, beta, case, nsteps, checkpoint_every):
+            c = case.copy()
+            c.A_override = float(A)
+            c.beta_recolor = float(beta)
+            s = Solver(c)
+            s.init_bubble_tanh()
+            s.set_populations(fr0.detach().cpu().numpy(), fb0.detach().cpu().numpy())
+            s.adjoint_record(nsteps, checkpoint_every)
+            fr, fb = s.populations()
+            ctx.solver = s
+            ctx.device, ctx.dtype = fr0.device, fr0.dtype
+            return torch.from_numpy(fr).to(fr0.device), torch.from_numpy(fb).to(fr0.device)
+
+        @staticmethod
+        def backward(ctx, g_fr, g_fb):
+            s = ctx.solver
+            if g_fr is None:
+                g_fr = torch.zeros((27,) + s.shape)
+            if g_fb is None:
+                g_fb = torch.zeros((27,) + s.shape)
+            lr, lb, (dA, dbeta) = s.adjoint_backward(g_fr.detach().cpu().numpy(), g_fb.detach().cpu().numpy())
+            s.close()
+            ctx.solver = None
+            t = lambda a: torch.from_numpy(a).to(device=ctx.device, dtype=ctx.dtype)  # noqa: E731
+            return (t(lr), t(lb), torch.tensor(dA, dtype=torch.float64), torch.tensor(dbeta, dtype=torch.float64),
+                    None, None, None)
+
+    return LBMSteps
+
+
+def bubble_centroid_height(fr: "torch.Tensor", fb: "torch.Tensor") -> "torch.Tensor":
+    """Height (y) of the blue component's centroid over the interior nodes: the
+    objective of the rising-bubble adjoint (differentiable in torch)."""
+    torch = _torch()
+    rb = fb[:, 1:-1, 1:-1, 1:-1].sum(0).double()
+    ny = rb.shape[1]
+    y = torch.arange(1, ny + 1, dtype=torch.float64).view(1, ny, 1)
+    return (rb * y).sum() / rb.sum()

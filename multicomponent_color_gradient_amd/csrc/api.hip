@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/mcg.h"
+#include "adjoint.hpp"
 #include "device.hpp"
 #include "fused.hpp"
 #include "kernels.hpp"
@@ -86,6 +87,34 @@ struct mcg_solver {
     int64_t start_step = 0, samples = 0;
     InitGeo geo{};
     size_t bytes = 0;
+
+    // adjoint: checkpoints of the recorded forward, the states of one segment,
+    // the adjoint fields and the FP64 parameter sums
+    bool stats_frozen = false;  // recomputation must not touch the statistics
+    bool adj_recorded = false;
+    int adj_nsteps = 0, adj_every = 0;
+    int64_t adj_step0 = 0;
+    std::vector<void *> adj_ckpt, adj_seg;
+    void *adj_keep[2] = {nullptr, nullptr};
+    float *adj_lam[2] = {nullptr, nullptr};
+    float *adj_lm = nullptr, *adj_lF = nullptr;
+    double *adj_acc = nullptr;
+    double adj_ms = 0.0;  // device time of the last backward sweep (recomputation + adjoint)
+
+    void adj_release() {
+        for (void *p : adj_ckpt) (void)hipFree(p);
+        for (void *p : adj_seg) (void)hipFree(p);
+        adj_ckpt.clear();
+        adj_seg.clear();
+        for (void *p : {adj_keep[0], adj_keep[1], (void *)adj_lam[0], (void *)adj_lam[1], (void *)adj_lm, (void *)adj_lF,
+                        (void *)adj_acc})
+            if (p) (void)hipFree(p);
+        adj_keep[0] = adj_keep[1] = nullptr;
+        adj_lam[0] = adj_lam[1] = nullptr;
+        adj_lm = adj_lF = nullptr;
+        adj_acc = nullptr;
+        adj_recorded = false;
+    }
 
     size_t N() const { return size_t(c.NX) * c.NY * c.NZ; }
     size_t nxz() const { return size_t(c.NX) * c.NZ; }
@@ -251,13 +280,13 @@ int do_init(mcg_solver *s, int kind) {
 
 int one_step(mcg_solver *s) {
     // update_radial_statistics_start, meanF/meanFieldsRT.cuh:173-189
-    if (!s->started && s->step >= s->c.NSTATS_START_STEP) {
+    if (!s->stats_frozen && !s->started && s->step >= s->c.NSTATS_START_STEP) {
         s->started = true;
         s->start_step = s->step;
         s->samples = 0;
     }
     const int64_t every = s->c.NSTATS_SAMPLE > 0 ? s->c.NSTATS_SAMPLE : 1;
-    const bool do_stats = s->started && ((s->step - s->start_step) % every == 0);
+    const bool do_stats = !s->stats_frozen && s->started && ((s->step - s->start_step) % every == 0);
     const DevBufs B = s->bufs(do_stats);
     if (s->kernel == MCG_KERNEL_TWOPASS) {
         int rc = launch_moments(s, s->cur);
@@ -289,7 +318,7 @@ int one_step(mcg_solver *s) {
         if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
         s->moments_valid = have_moments;
     }
-    if (do_stats) ++s->samples;
+    if (do_stats) ++s->samples;  // never during adjoint recomputation (stats_frozen)
     if (s->inlet_reset_pending) {
         hipLaunchKernelGGL(k_inlet_ghost_reset, dim3(unsigned((s->c.NX - 2 + 255) / 256), unsigned(s->c.NZ - 2)), dim3(256), 0,
                            s->stream, s->P, s->grr, s->grb);
@@ -445,6 +474,7 @@ int mcg_destroy(mcg_solver *s) {
                     (void *)s->red_partials, (void *)s->red_out})
         if (p) (void)hipFree(p);
     s->release_staging();
+    s->adj_release();
     delete s;
     return MCG_OK;
 }
@@ -749,6 +779,196 @@ int mcg_device_bytes(const mcg_solver *s, uint64_t *nbytes) {
     if (int rc = check_solver(s)) return rc;
     if (!nbytes) return fail(MCG_E_INVALID, "null pointer");
     *nbytes = s->bytes;
+    return MCG_OK;
+}
+
+// ---------------------------------------------------------------- adjoint
+
+static int adj_supported(const mcg_case &c, int storage) {
+    if (c.y_mode != MCG_Y_PERIODIC || !c.wrap_gradient)
+        return fail(MCG_E_UNSUPPORTED, "the adjoint covers the periodic domain (y_mode periodic, wrap_gradient 1)");
+    if (c.viscosity_mode != MCG_VISC_SINGLE) return fail(MCG_E_UNSUPPORTED, "the adjoint covers the single relaxation rate");
+    if (storage != MCG_STORE_FP32) return fail(MCG_E_UNSUPPORTED, "the adjoint runs on FP32 populations");
+    return MCG_OK;
+}
+
+int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every) {
+    if (int rc = check_ready(s)) return rc;
+    if (int rc = adj_supported(s->c, s->storage)) return rc;
+    if (nsteps < 1 || checkpoint_every < 1) return fail(MCG_E_INVALID, "nsteps and checkpoint_every must be >= 1");
+    s->adj_release();
+    const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
+    for (int k = 0; k < nck; ++k) {
+        void *p = nullptr;
+        if (hipMalloc(&p, s->pop_bytes()) != hipSuccess) {
+            s->adj_release();
+            return fail(MCG_E_NOMEM, "adjoint checkpoint allocation failed");
+        }
+        s->adj_ckpt.push_back(p);
+    }
+    s->adj_step0 = s->step;
+    for (int k = 0; k < nsteps; ++k) {
+        if (k % checkpoint_every == 0)
+            HIP_TRY(hipMemcpyAsync(s->adj_ckpt[k / checkpoint_every], s->f[s->cur], s->pop_bytes(), hipMemcpyDeviceToDevice,
+                                   s->stream));
+        const int rc = one_step(s);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->adj_nsteps = nsteps;
+    s->adj_every = checkpoint_every;
+    s->adj_recorded = true;
+    return MCG_OK;
+}
+
+static int adj_alloc(mcg_solver *s) {
+    auto get = [&](void **p, size_t nb) -> bool { return *p || hipMalloc(p, nb) == hipSuccess; };
+    bool ok = true;
+    for (int b = 0; b < 2; ++b) {
+        ok = ok && get(&s->adj_keep[b], s->pop_bytes());
+        ok = ok && get((void **)&s->adj_lam[b], s->pop_bytes());
+    }
+    ok = ok && get((void **)&s->adj_lm, size_t(11) * s->P.plane * sizeof(float));
+    ok = ok && get((void **)&s->adj_lF, size_t(3) * s->P.plane * sizeof(float));
+    ok = ok && get((void **)&s->adj_acc, 2 * sizeof(double));
+    while (ok && int(s->adj_seg.size()) < s->adj_every) {
+        void *p = nullptr;
+        ok = hipMalloc(&p, s->pop_bytes()) == hipSuccess;
+        if (ok) s->adj_seg.push_back(p);
+    }
+    return ok ? MCG_OK : fail(MCG_E_NOMEM, "adjoint buffer allocation failed");
+}
+
+int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb, float *out_lam_fr, float *out_lam_fb,
+                         double *dparams) {
+    if (int rc = check_ready(s)) return rc;
+    if (!s->adj_recorded) return fail(MCG_E_STATE, "no recorded forward (call mcg_adjoint_record first)");
+    if (s->step != s->adj_step0 + s->adj_nsteps) return fail(MCG_E_STATE, "the solver advanced after mcg_adjoint_record");
+    if (!lam_fr || !lam_fb) return fail(MCG_E_INVALID, "null adjoint seed");
+    if (int rc = adj_alloc(s)) return rc;
+    const size_t n = size_t(Q) * s->N();
+    if (int rc = s->ensure_staging(2 * n * sizeof(float))) return rc;
+    // keep the final state (both buffers: the last moments come from the previous one)
+    const int cur0 = s->cur;
+    for (int b = 0; b < 2; ++b)
+        HIP_TRY(hipMemcpyAsync(s->adj_keep[b], s->f[b], s->pop_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->staging, lam_fr, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->staging + n, lam_fb, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->adj_lam[0], 0, s->pop_bytes(), s->stream));
+    hipLaunchKernelGGL(k_rows_from_ref, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->staging, s->staging + n,
+                       s->adj_lam[0]);
+    LAUNCH_CHECK("k_rows_from_ref");
+    HIP_TRY(hipMemsetAsync(s->adj_acc, 0, 2 * sizeof(double), s->stream));
+    s->stats_frozen = true;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    int a = 0;
+    int rc = MCG_OK;
+    const int nck = int(s->adj_ckpt.size());
+    for (int seg = nck - 1; seg >= 0 && rc == MCG_OK; --seg) {
+        const int n0 = seg * s->adj_every, n1 = std::min(n0 + s->adj_every, s->adj_nsteps);
+        HIP_TRY(hipMemcpyAsync(s->f[s->cur], s->adj_ckpt[seg], s->pop_bytes(), hipMemcpyDeviceToDevice, s->stream));
+        s->step = s->adj_step0 + n0;
+        for (int k = n0; k < n1 && rc == MCG_OK; ++k) {
+            HIP_TRY(hipMemcpyAsync(s->adj_seg[k - n0], s->f[s->cur], s->pop_bytes(), hipMemcpyDeviceToDevice, s->stream));
+            rc = one_step(s);
+        }
+        for (int k = n1 - 1; k >= n0 && rc == MCG_OK; --k) {
+            hipLaunchKernelGGL(k_moments<MCG_STORE_FP32>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P,
+                               s->adj_seg[k - n0], s->bin[s->cur], s->bout[s->cur], s->mom);
+            LAUNCH_CHECK("k_moments (adjoint)");
+            hipLaunchKernelGGL(k_adj_node, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_lam[a],
+                               s->adj_lm, s->adj_lF, s->adj_acc);
+            LAUNCH_CHECK("k_adj_node");
+            hipLaunchKernelGGL(k_adj_pop, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_lm,
+                               s->adj_lF, s->adj_lam[a ^ 1]);
+            LAUNCH_CHECK("k_adj_pop");
+            a ^= 1;
+        }
+    }
+    s->stats_frozen = false;
+    (void)hipEventRecord(e1, s->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->adj_ms = double(ms);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(s->staging, 0, 2 * n * sizeof(float), s->stream));
+    hipLaunchKernelGGL(k_rows_to_ref, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->adj_lam[a], s->staging,
+                       s->staging + n);
+    LAUNCH_CHECK("k_rows_to_ref");
+    if (out_lam_fr) HIP_TRY(hipMemcpyAsync(out_lam_fr, s->staging, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    if (out_lam_fb) HIP_TRY(hipMemcpyAsync(out_lam_fb, s->staging + n, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    double acc[2] = {0.0, 0.0};
+    HIP_TRY(hipMemcpyAsync(acc, s->adj_acc, sizeof(acc), hipMemcpyDeviceToHost, s->stream));
+    // restore the final state
+    s->cur = cur0;
+    for (int b = 0; b < 2; ++b)
+        HIP_TRY(hipMemcpyAsync(s->f[b], s->adj_keep[b], s->pop_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->step = s->adj_step0 + s->adj_nsteps;
+    s->moments_valid = false;
+    if (dparams) {
+        dparams[0] = acc[0];
+        dparams[1] = acc[1];
+    }
+    return MCG_OK;
+}
+
+int mcg_adjoint_last_ms(const mcg_solver *s, double *ms) {
+    if (int rc = check_solver(s)) return rc;
+    if (!ms) return fail(MCG_E_INVALID, "null pointer");
+    *ms = s->adj_ms;
+    return MCG_OK;
+}
+
+int mcg_adjoint_release(mcg_solver *s) {
+    if (int rc = check_solver(s)) return rc;
+    s->adj_release();
+    return MCG_OK;
+}
+
+int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *lam_fr,
+                    const double *lam_fb, double *out_fr, double *out_fb, double *out_lam_fr, double *out_lam_fb,
+                    double *dparams) {
+    if (!c || !fr0 || !fb0) return fail(MCG_E_INVALID, "null pointer");
+    if (int rc = mcg_case_validate(c)) return rc;
+    if (int rc = adj_supported(*c, MCG_STORE_FP32)) return rc;
+    if (c->sponge_K != 0.0) return fail(MCG_E_UNSUPPORTED, "the CPU adjoint assumes a uniform relaxation rate (sponge_K = 0)");
+    if (nsteps < 0) return fail(MCG_E_INVALID, "nsteps < 0");
+    CpuAdjoint A;
+    A.NX = c->NX;
+    A.NY = c->NY;
+    A.NZ = c->NZ;
+    A.N = size_t(c->NX) * c->NY * c->NZ;
+    A.omega = 1.0 / c->taub;
+    A.A = c->A_override > 0.0 ? c->A_override : c->sigma / (4.0 / 9.0) / c->taub;
+    A.beta = c->beta_recolor;
+    const size_t n = size_t(Q) * A.N;
+    std::vector<std::vector<double>> R(size_t(nsteps) + 1, std::vector<double>(n)), B(size_t(nsteps) + 1, std::vector<double>(n));
+    std::copy(fr0, fr0 + n, R[0].begin());
+    std::copy(fb0, fb0 + n, B[0].begin());
+    for (int k = 0; k < nsteps; ++k) A.step(R[k].data(), B[k].data(), R[k + 1].data(), B[k + 1].data());
+    if (out_fr) std::copy(R[nsteps].begin(), R[nsteps].end(), out_fr);
+    if (out_fb) std::copy(B[nsteps].begin(), B[nsteps].end(), out_fb);
+    if (!lam_fr || !lam_fb) return MCG_OK;
+    std::vector<double> lr(lam_fr, lam_fr + n), lb(lam_fb, lam_fb + n), tr(n), tb(n);
+    double dA = 0.0, dbeta = 0.0;
+    for (int k = nsteps - 1; k >= 0; --k) {
+        A.adjoint(R[k].data(), B[k].data(), lr.data(), lb.data(), tr.data(), tb.data(), dA, dbeta);
+        lr.swap(tr);
+        lb.swap(tb);
+    }
+    if (out_lam_fr) std::copy(lr.begin(), lr.end(), out_lam_fr);
+    if (out_lam_fb) std::copy(lb.begin(), lb.end(), out_lam_fb);
+    if (dparams) {
+        dparams[0] = dA;
+        dparams[1] = dbeta;
+    }
     return MCG_OK;
 }
 

@@ -147,6 +147,32 @@ class Solver:
         f = ctypes.c_float
         check(self.lib.mcg_set_populations(self._h, as_ptr(fr, f), as_ptr(fb, f)))
 
+    # ---- adjoint (an extension; periodic domain, FP32 populations)
+    def adjoint_record(self, nsteps: int, checkpoint_every: int) -> None:
+        """Run nsteps steps from the current state, checkpointing every checkpoint_every steps."""
+        check(self.lib.mcg_adjoint_record(self._h, int(nsteps), int(checkpoint_every)))
+
+    def adjoint_backward(self, lam_fr: np.ndarray, lam_fb: np.ndarray):
+        """dJ/df at the final recorded state -> (dJ/df_r, dJ/df_b at the initial state, (dJ/dA, dJ/dbeta))."""
+        lam_fr = np.ascontiguousarray(lam_fr, dtype=np.float32).reshape((27,) + self.shape)
+        lam_fb = np.ascontiguousarray(lam_fb, dtype=np.float32).reshape((27,) + self.shape)
+        out_r = np.empty((27,) + self.shape, np.float32)
+        out_b = np.empty((27,) + self.shape, np.float32)
+        dp = np.zeros(2, np.float64)
+        f = ctypes.c_float
+        check(self.lib.mcg_adjoint_backward(self._h, as_ptr(lam_fr, f), as_ptr(lam_fb, f), as_ptr(out_r, f),
+                                            as_ptr(out_b, f), as_ptr(dp, ctypes.c_double)))
+        return out_r, out_b, (float(dp[0]), float(dp[1]))
+
+    def adjoint_last_ms(self) -> float:
+        """Device time of the last adjoint_backward sweep (recomputation + adjoint kernels)."""
+        ms = ctypes.c_double()
+        check(self.lib.mcg_adjoint_last_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def adjoint_release(self) -> None:
+        check(self.lib.mcg_adjoint_release(self._h))
+
     # ---- output files (io/save_data.cuh, meanF/meanFieldsRT.cuh)
     def write_vti(self, path: str) -> None:
         check(self.lib.mcg_write_vti(self._h, os.fsencode(path)))
