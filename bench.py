@@ -84,6 +84,33 @@ def copy_probe(nbytes: int, reps: int = 3) -> None:
         hip.hipFree(b)
 
 
+def adjoint_ratio(steps: int, every: int) -> dict:
+    """Adjoint/forward step-cost ratio on a 192x384x192 periodic bubble (R=16,
+    tau 0.8, A=1e-3, beta 0.7): device time of the backward sweep (checkpoint
+    recomputation + adjoint kernels) over the device time of the same number
+    of plain forward steps."""
+    import numpy as np
+
+    import multicomponent_color_gradient_amd as m
+
+    c = m.periodic_bubble_case(192, radius=16.0, tau=0.8, A=1e-3, beta=0.7)
+    c.NY = 384
+    c.bubble_y0 = (c.NY - 1) / 2.0
+    s = m.Solver(c)
+    s.init_bubble_tanh()
+    s.step(10)
+    fwd_ms = s.time_steps(steps)
+    s.adjoint_record(steps, every)
+    lam = np.zeros((27,) + s.shape, np.float32)
+    lam[:, 1:-1, 1:-1, 1:-1] = 1e-3
+    s.adjoint_backward(lam, lam)
+    bwd_ms = s.adjoint_last_ms()
+    s.close()
+    return {"workload": "periodic bubble 192x384x192, R=16, tau 0.8, A=1e-3, beta 0.7", "steps": steps,
+            "checkpoint_every": every, "forward_ms": round(fwd_ms, 3), "backward_ms": round(bwd_ms, 3),
+            "ratio": round(bwd_ms / fwd_ms, 3)}
+
+
 def cpu_baseline(case, init: str, budget_s: float):
     """FP64 OpenMP oracle on the same grid: 1 warm-up step, then steps until budget_s."""
     from oracle import oracle as orc
@@ -123,6 +150,10 @@ def main():
     ap.add_argument("--kernel", default="fused")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-offset comparison run")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a device buffer of this size 3 times with hipMemcpy "
                          "(a plain copy kernel of known size that calibrates rocprofv3's FETCH_SIZE/WRITE_SIZE)")
@@ -136,31 +167,38 @@ def main():
     import multicomponent_color_gradient_amd as m
 
     case, init = make_case(args.workload)
-    s = m.Solver(case, storage=args.storage, kernel=args.kernel)
-    getattr(s, "init_" + init)()
-    print(f"bench: {args.workload} {case.NX}x{case.NY}x{case.NZ} storage={args.storage} kernel={args.kernel} "
-          f"warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
-    s.step(args.warmup)
-    s.synchronize()
-    t0 = time.perf_counter()
-    ms = s.time_steps(args.steps)
-    wall = time.perf_counter() - t0
-    s.check_health()
-    cells = case.NX * case.NY * case.NZ
-    interior = m.interior_nodes(case)
-    sec = ms / 1e3
-    mlups = args.steps * cells / sec / 1e6
-    mlups_int = args.steps * interior / sec / 1e6
-    avg_ms = ms / args.steps
-    bpl = BYTES_PER_LU[args.storage]
-    achieved = bpl * interior / (avg_ms / 1e3) / 1e9
-    traffic = load_traffic(args.workload, args.storage)
-    s.close()
+
+    def measure(storage: str) -> dict:
+        s = m.Solver(case, storage=storage, kernel=args.kernel)
+        getattr(s, "init_" + init)()
+        print(f"bench: {args.workload} {case.NX}x{case.NY}x{case.NZ} storage={storage} kernel={args.kernel} "
+              f"warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+        s.step(args.warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        ms = s.time_steps(args.steps)
+        wall = time.perf_counter() - t0
+        s.check_health()
+        s.close()
+        cells = case.NX * case.NY * case.NZ
+        interior = m.interior_nodes(case)
+        avg_ms = ms / args.steps
+        bpl = BYTES_PER_LU[storage]
+        achieved = bpl * interior / (avg_ms / 1e3) / 1e9
+        return {"mlups": args.steps * cells / (ms / 1e3) / 1e6, "mlups_int": args.steps * interior / (ms / 1e3) / 1e6,
+                "avg_ms": avg_ms, "wall": wall, "bpl": bpl, "achieved": achieved,
+                "traffic": load_traffic(args.workload, storage)}
+
+    r = measure(args.storage)
+    mlups, mlups_int, avg_ms, wall = r["mlups"], r["mlups_int"], r["avg_ms"], r["wall"]
+    bpl, achieved, traffic = r["bpl"], r["achieved"], r["traffic"]
+    # the FP16-offset comparison of the metric, on the same workload
+    r16 = measure("fp16") if args.storage == "fp32" and not args.no_fp16 else None
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib << 20)
 
     out = {
-        "metric": "MLUPS (fused step, % HBM roofline at 432 B/LU FP32)",
+        "metric": "MLUPS + % HBM roofline (FP32 vs FP16-offset); adjoint/forward step cost ratio",
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -184,6 +222,14 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_lu": bpl, "kernel_ms": round(avg_ms, 5)},
     }
+    if r16 is not None:
+        out["fp16_offset"] = {"value": round(r16["mlups"], 2), "unit": "MLUPS", "ms_per_step": round(r16["avg_ms"], 5),
+                              "speedup_vs_fp32": round(r16["mlups"] / mlups, 3),
+                              "roofline": {"achieved": round(r16["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                           "frac": round(r16["achieved"] / HBM_PEAK_GBS, 4), "traffic": r16["traffic"],
+                                           "bytes_per_lu": r16["bpl"]}}
+    if not args.no_adjoint and args.storage == "fp32":
+        out["adjoint"] = adjoint_ratio(args.adjoint_steps, args.adjoint_every)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, init, args.cpu_seconds)
     else:

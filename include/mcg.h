@@ -221,7 +221,7 @@ int mcg_checkpoint_load(mcg_solver *s, const void *host_buf, uint64_t nbytes);
 
 /* Run nsteps forward steps from the current state and keep a copy of the
  * populations every checkpoint_every steps (device memory: ceil(nsteps/K) +
- * K + 4 population sets during mcg_adjoint_backward). */
+ * K + 3 population sets during mcg_adjoint_backward). */
 int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every);
 
 /* Given lam = dJ/df at the final state (reference layout, 27 x NX*NY*NZ per

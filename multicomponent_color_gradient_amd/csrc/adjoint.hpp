@@ -39,11 +39,26 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+#ifndef MCG_ADJ_WAVES_PER_EU
+#define MCG_ADJ_WAVES_PER_EU 4
+#endif
+
+// psi of every interior node into one plane (the colour gradients of the
+// adjoint read 26 neighbours each)
+__global__ __launch_bounds__(256) void k_psi_plane(DevParams P, const float *__restrict__ mom, float *__restrict__ ps) {
+    MCG_INTERIOR_XYZ(P, x, y, z)
+    const uint32_t o = midx(P, x, y, z);
+    ps[o] = mom_psi(P, mom, o);
+}
+
 // Adjoint of the collision at every interior node: lam_m (11 planes),
-// lam_F (3 planes), and dJ/dA, dJ/dbeta added to acc[0..1].
-__global__ __launch_bounds__(256) void k_adj_node(DevParams P, const float *__restrict__ mom,
-                                                  const float *__restrict__ lam_in, float *__restrict__ lam_m,
-                                                  float *__restrict__ lam_F, double *acc) {
+// lam_F (3 planes), and the workgroup's sums of dJ/dA, dJ/dbeta in
+// part[2 b], part[2 b + 1] (b = linear workgroup id; reduced by k_adj_sum in
+// a fixed order, so the FP64 parameter gradients are deterministic).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MCG_ADJ_WAVES_PER_EU)))
+void k_adj_node(DevParams P, const float *__restrict__ mom, const float *__restrict__ ps,
+                const float *__restrict__ lam_in, float *__restrict__ lam_m, float *__restrict__ lam_F, double *part) {
+    __shared__ double wsum[2][4];
     const int x = 1 + int(blockIdx.x) * 64 + int(threadIdx.x);
     const int y = 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
     const int z = 1 + int(blockIdx.z);
@@ -56,19 +71,17 @@ __global__ __launch_bounds__(256) void k_adj_node(DevParams P, const float *__re
         float Fx = 0.f, Fy = 0.f, Fz = 0.f;
         auto phi = [&](auto Ic) -> float {
             constexpr int I = decltype(Ic)::value;
-            return mom_psi(P, mom, midx(P, wrapN(x + kCX[I], P.NX), wrapN(y + kCY[I], P.NY), wrapN(z + kCZ[I], P.NZ)));
+            return ps[midx(P, wrapN(x + kCX[I], P.NX), wrapN(y + kCY[I], P.NY), wrapN(z + kCZ[I], P.NZ))];
         };
         color_gradient(phi, Fx, Fy, Fz);
-        float mu_r[Q], mu_b[Q];
-        unroll<0, Q>([&](auto Ic) {
-            constexpr int I = decltype(Ic)::value;
-            const int xs = wrapN(x + kCX[I], P.NX), ys = wrapN(y + kCY[I], P.NY), zs = wrapN(z + kCZ[I], P.NZ);
-            mu_r[I] = lam_in[nrow(P, 0, I, xs, ys, zs)];
-            mu_b[I] = lam_in[nrow(P, 1, I, xs, ys, zs)];
-        });
         const float omega = omega_node(P, y, m[0], m[1]);
         const AdjNode<float> n = adj_setup<float>(m, Fx, Fy, Fz, A_node(P, m[0], m[1]), P.beta, omega);
-        const AdjOut<float> a = adj_vjp<float>(n, mu_r, mu_b);
+        // mu_i(x) = lambda_i(x + c_i), loaded where the pair needs it
+        const AdjOut<float> a = adj_vjp_fn<float>(n, [&](int i, float &r, float &b) {
+            const uint32_t q = pidx(P, wrapN(x + kCX[i], P.NX), wrapN(y + kCY[i], P.NY), wrapN(z + kCZ[i], P.NZ));
+            r = lam_in[uint32_t(i) * uint32_t(P.PX) + q];
+            b = lam_in[uint32_t(Q + i) * uint32_t(P.PX) + q];
+        });
         for (int k = 0; k < 11; ++k) lam_m[size_t(k) * P.plane + o] = a.m[k];
         for (int k = 0; k < 3; ++k) lam_F[size_t(k) * P.plane + o] = a.F[k];
         gA = double(a.A);
@@ -77,8 +90,38 @@ __global__ __launch_bounds__(256) void k_adj_node(DevParams P, const float *__re
     gA = wave_sum(gA);
     gB = wave_sum(gB);
     if (threadIdx.x == 0) {
-        atomicAdd(acc + 0, gA);
-        atomicAdd(acc + 1, gB);
+        wsum[0][threadIdx.y] = gA;
+        wsum[1][threadIdx.y] = gB;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const size_t b = size_t(blockIdx.x) + size_t(gridDim.x) * (size_t(blockIdx.y) + size_t(gridDim.y) * blockIdx.z);
+        part[2 * b] = (wsum[0][0] + wsum[0][1]) + (wsum[0][2] + wsum[0][3]);
+        part[2 * b + 1] = (wsum[1][0] + wsum[1][1]) + (wsum[1][2] + wsum[1][3]);
+    }
+}
+
+// acc[0..1] += the sums of the nb workgroup partials (one workgroup, fixed order)
+__global__ __launch_bounds__(256) void k_adj_sum(const double *__restrict__ part, size_t nb, double *acc) {
+    __shared__ double sh[2][256];
+    double a = 0.0, b = 0.0;
+    for (size_t k = threadIdx.x; k < nb; k += 256) {
+        a += part[2 * k];
+        b += part[2 * k + 1];
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        acc[0] += sh[0][0];
+        acc[1] += sh[1][0];
     }
 }
 

@@ -190,17 +190,22 @@ struct AdjOut {
     T A, beta;
 };
 
-template <class T>
-MCG_HD AdjOut<T> adj_vjp(const AdjNode<T> &n, const T *mu_r, const T *mu_b) {
+// mu(i, r, b) yields the adjoint of the two colours of direction i; it is
+// called once per direction, in pair order, so that callers can load it on
+// demand.
+template <class T, class MuFn>
+MCG_HD AdjOut<T> adj_vjp_fn(const AdjNode<T> &n, MuFn &&mu) {
     T l_rho = 0, l_aR = 0, l_aB = 0, l_usq = 0, l_sB = 0, l_s2 = 0, l_rc = 0, l_tr = 0;
     T l_u[3] = {0, 0, 0}, l_P[6] = {0, 0, 0, 0, 0, 0}, l_t[7] = {0, 0, 0, 0, 0, 0, 0}, l_F[3] = {0, 0, 0};
     const T e0 = n.rho * (T(1) - T(1.5) * n.usq);
     // rest direction
     {
         const T g0 = wT<T>(0) * (e0 + n.q2 * n.tr + T(1.25) * n.sB);
-        const T l_g0 = n.aR * mu_r[0] + n.aB * mu_b[0];
-        l_aR += g0 * mu_r[0];
-        l_aB += g0 * mu_b[0];
+        T mr0, mb0;
+        mu(0, mr0, mb0);
+        const T l_g0 = n.aR * mr0 + n.aB * mb0;
+        l_aR += g0 * mr0;
+        l_aB += g0 * mb0;
         const T l_E0 = wT<T>(0) * l_g0;
         l_rho += (T(1) - T(1.5) * n.usq) * l_E0;
         l_usq += -T(1.5) * n.rho * l_E0;
@@ -217,11 +222,14 @@ MCG_HD AdjOut<T> adj_vjp(const AdjNode<T> &n, const T *mu_r, const T *mu_b) {
         const T O = n.rho * cu * (T(3) - T(4.5) * n.usq) + T(4.5) * n.rho * cu * cu * cu + n.q3 * a3;
         const T wi = wT<T>(i);
         const T gp = wi * (E + O), gm = wi * (E - O);
-        const T l_gp = n.aR * mu_r[i] + n.aB * mu_b[i];
-        const T l_gm = n.aR * mu_r[j] + n.aB * mu_b[j];
-        l_aR += gp * mu_r[i] + gm * mu_r[j];
-        l_aB += gp * mu_b[i] + gm * mu_b[j];
-        const T l_D = wi * (mu_r[i] - mu_b[i] - mu_r[j] + mu_b[j]);
+        T mri, mbi, mrj, mbj;
+        mu(i, mri, mbi);
+        mu(j, mrj, mbj);
+        const T l_gp = n.aR * mri + n.aB * mbi;
+        const T l_gm = n.aR * mrj + n.aB * mbj;
+        l_aR += gp * mri + gm * mrj;
+        l_aB += gp * mbi + gm * mbj;
+        const T l_D = wi * (mri - mbi - mrj + mbj);
         const T l_E = wi * (l_gp + l_gm);
         const T l_O = wi * (l_gp - l_gm);
         // E
@@ -318,6 +326,14 @@ MCG_HD AdjOut<T> adj_vjp(const AdjNode<T> &n, const T *mu_r, const T *mu_b) {
     o.F[1] = l_F[1];
     o.F[2] = l_F[2];
     return o;
+}
+
+template <class T>
+MCG_HD AdjOut<T> adj_vjp(const AdjNode<T> &n, const T *mu_r, const T *mu_b) {
+    return adj_vjp_fn<T>(n, [&](int i, T &r, T &b) {
+        r = mu_r[i];
+        b = mu_b[i];
+    });
 }
 
 // Adjoint of the moments pass (lbm.cuh:82-165) at one node: from the moments

@@ -97,8 +97,8 @@ struct mcg_solver {
     std::vector<void *> adj_ckpt, adj_seg;
     void *adj_keep[2] = {nullptr, nullptr};
     float *adj_lam[2] = {nullptr, nullptr};
-    float *adj_lm = nullptr, *adj_lF = nullptr;
-    double *adj_acc = nullptr;
+    float *adj_lm = nullptr, *adj_lF = nullptr, *adj_ps = nullptr;
+    double *adj_acc = nullptr, *adj_part = nullptr;
     double adj_ms = 0.0;  // device time of the last backward sweep (recomputation + adjoint)
 
     void adj_release() {
@@ -107,12 +107,12 @@ struct mcg_solver {
         adj_ckpt.clear();
         adj_seg.clear();
         for (void *p : {adj_keep[0], adj_keep[1], (void *)adj_lam[0], (void *)adj_lam[1], (void *)adj_lm, (void *)adj_lF,
-                        (void *)adj_acc})
+                        (void *)adj_ps, (void *)adj_acc, (void *)adj_part})
             if (p) (void)hipFree(p);
         adj_keep[0] = adj_keep[1] = nullptr;
         adj_lam[0] = adj_lam[1] = nullptr;
-        adj_lm = adj_lF = nullptr;
-        adj_acc = nullptr;
+        adj_lm = adj_lF = adj_ps = nullptr;
+        adj_acc = adj_part = nullptr;
         adj_recorded = false;
     }
 
@@ -821,6 +821,21 @@ int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every) {
     return MCG_OK;
 }
 
+// one step from population buffer src into dst (the solver's own buffers are
+// put back afterwards; the step counter advances)
+static int step_between(mcg_solver *s, void *src, void *dst) {
+    void *f0 = s->f[0], *f1 = s->f[1];
+    const int cur = s->cur;
+    s->f[0] = src;
+    s->f[1] = dst;
+    s->cur = 0;
+    const int rc = one_step(s);
+    s->f[0] = f0;
+    s->f[1] = f1;
+    s->cur = cur;
+    return rc;
+}
+
 static int adj_alloc(mcg_solver *s) {
     auto get = [&](void **p, size_t nb) -> bool { return *p || hipMalloc(p, nb) == hipSuccess; };
     bool ok = true;
@@ -830,8 +845,12 @@ static int adj_alloc(mcg_solver *s) {
     }
     ok = ok && get((void **)&s->adj_lm, size_t(11) * s->P.plane * sizeof(float));
     ok = ok && get((void **)&s->adj_lF, size_t(3) * s->P.plane * sizeof(float));
+    ok = ok && get((void **)&s->adj_ps, size_t(s->P.plane) * sizeof(float));
     ok = ok && get((void **)&s->adj_acc, 2 * sizeof(double));
-    while (ok && int(s->adj_seg.size()) < s->adj_every) {
+    const dim3 g = s->grid_interior();
+    ok = ok && get((void **)&s->adj_part, 2 * sizeof(double) * size_t(g.x) * g.y * g.z);
+    // a segment's first state is its checkpoint; the others are recomputed into these
+    while (ok && int(s->adj_seg.size()) < s->adj_every - 1) {
         void *p = nullptr;
         ok = hipMalloc(&p, s->pop_bytes()) == hipSuccess;
         if (ok) s->adj_seg.push_back(p);
@@ -869,19 +888,23 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     const int nck = int(s->adj_ckpt.size());
     for (int seg = nck - 1; seg >= 0 && rc == MCG_OK; --seg) {
         const int n0 = seg * s->adj_every, n1 = std::min(n0 + s->adj_every, s->adj_nsteps);
-        HIP_TRY(hipMemcpyAsync(s->f[s->cur], s->adj_ckpt[seg], s->pop_bytes(), hipMemcpyDeviceToDevice, s->stream));
+        // states n0 .. n1-1 of the segment: the checkpoint, then recomputed in place
+        auto state = [&](int k) -> void * { return k == n0 ? s->adj_ckpt[seg] : s->adj_seg[k - n0 - 1]; };
         s->step = s->adj_step0 + n0;
-        for (int k = n0; k < n1 && rc == MCG_OK; ++k) {
-            HIP_TRY(hipMemcpyAsync(s->adj_seg[k - n0], s->f[s->cur], s->pop_bytes(), hipMemcpyDeviceToDevice, s->stream));
-            rc = one_step(s);
-        }
+        for (int k = n0; k + 1 < n1 && rc == MCG_OK; ++k) rc = step_between(s, state(k), state(k + 1));
         for (int k = n1 - 1; k >= n0 && rc == MCG_OK; --k) {
-            hipLaunchKernelGGL(k_moments<MCG_STORE_FP32>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P,
-                               s->adj_seg[k - n0], s->bin[s->cur], s->bout[s->cur], s->mom);
+            hipLaunchKernelGGL(k_moments<MCG_STORE_FP32>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, state(k),
+                               s->bin[s->cur], s->bout[s->cur], s->mom);
             LAUNCH_CHECK("k_moments (adjoint)");
-            hipLaunchKernelGGL(k_adj_node, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_lam[a],
-                               s->adj_lm, s->adj_lF, s->adj_acc);
+            hipLaunchKernelGGL(k_psi_plane, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_ps);
+            LAUNCH_CHECK("k_psi_plane");
+            const dim3 g = s->grid_interior();
+            hipLaunchKernelGGL(k_adj_node, g, dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_ps, s->adj_lam[a],
+                               s->adj_lm, s->adj_lF, s->adj_part);
             LAUNCH_CHECK("k_adj_node");
+            hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, s->stream, s->adj_part, size_t(g.x) * g.y * g.z,
+                               s->adj_acc);
+            LAUNCH_CHECK("k_adj_sum");
             hipLaunchKernelGGL(k_adj_pop, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_lm,
                                s->adj_lF, s->adj_lam[a ^ 1]);
             LAUNCH_CHECK("k_adj_pop");
