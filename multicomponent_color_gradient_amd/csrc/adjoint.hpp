@@ -154,8 +154,8 @@ __global__ __launch_bounds__(256) void k_adj_pop(DevParams P, const float *__res
     float lr[Q], lb[Q];
     adj_moments<float>(m, lam, lr, lb);
     for (int i = 0; i < Q; ++i) {
-        lam_out[nrow(P, 0, i, x, y, z)] = lr[i];
-        lam_out[nrow(P, 1, i, x, y, z)] = lb[i];
+        __builtin_nontemporal_store(lr[i], &lam_out[nrow(P, 0, i, x, y, z)]);
+        __builtin_nontemporal_store(lb[i], &lam_out[nrow(P, 1, i, x, y, z)]);
     }
 }
 

@@ -83,3 +83,38 @@ def test_16bit_fused_equals_twopass(mcg, storage):
     b.step(20)
     pa, pb = a.populations(), b.populations()
     assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1])
+
+
+def mean_profile_errors(mcg, storage, steps=120):
+    """Radial mean u_y profile <u_y>(y, r) = sum_uy / count of a perturbed jet
+    against the FP64 oracle: max |d <u_y>| / max |<u_y>| over the sampled bins,
+    and the relative error of the total mass (rho_r + rho_b)."""
+    from oracle.oracle import Oracle
+
+    case = mcg.perturbed_jet_case(34, 48, 34, jet_radius=6.0)
+    case.NSTATS_START_STEP = 20
+    case.NSTATS_SAMPLE = 2
+    s = mcg.Solver(case, storage=storage)
+    s.init_jet()
+    s.step(steps)
+    o = Oracle(case)
+    o.init_jet()
+    o.step(steps)
+    g, r = s.radial(), o.radial()
+    assert np.array_equal(g["count"], r["count"])
+    mask = r["count"] > 0
+    mg = g["sum_uy"][mask] / g["count"][mask]
+    mr = r["sum_uy"][mask] / r["count"][mask]
+    prof = float(np.max(np.abs(mg - mr)) / np.max(np.abs(mr)))
+    gs, os_ = s.global_sums(), o.global_sums()
+    mass = float(abs((gs[0] + gs[1]) - (os_[0] + os_[1])) / (os_[0] + os_[1]))
+    return prof, mass
+
+
+@pytest.mark.parametrize("storage,bound", [("fp32", 1e-4), ("fp16", 1e-3), ("bf16", 1e-2)])
+def test_jet_mean_profile_error(mcg, storage, bound):
+    """Mean u_y profiles of the jet: FP16-offset storage within 1e-3 of the FP64
+    oracle (BF16 within 1e-2, FP32 within 1e-4); total mass within 1e-5."""
+    prof, mass = mean_profile_errors(mcg, storage)
+    assert prof <= bound, f"{storage}: mean-profile error {prof:.3e}"
+    assert mass <= 1e-5, f"{storage}: mass error {mass:.3e}"
