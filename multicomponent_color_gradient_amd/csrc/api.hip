@@ -118,7 +118,7 @@ struct mcg_solver {
 
     size_t N() const { return size_t(c.NX) * c.NY * c.NZ; }
     size_t nxz() const { return size_t(c.NX) * c.NZ; }
-    size_t pop_bytes() const { return size_t(storage == MCG_STORE_FP32 ? 2 * Q : Q) * P.plane * sizeof(float); }
+    size_t pop_bytes() const { return size_t(P.rowstride) * size_t(P.IY) * size_t(P.IZ) * sizeof(float); }
     size_t bnd_bytes() const { return size_t(18) * nxz() * sizeof(float); }
     size_t gB_bytes() const { return size_t(5) * nxz() * sizeof(float); }
     size_t bins() const { return size_t(c.NY) * P.NR; }
@@ -165,6 +165,13 @@ struct mcg_solver {
 namespace {
 
 // derived constants of constants.cuh:59-76 for a case
+// words of padding after each (y,z) row block of the populations: concurrent
+// workgroups then start their rows at addresses spread over the memory channels
+#ifndef MCG_ROW_PAD
+#define MCG_ROW_PAD 0
+#endif
+constexpr uint32_t kRowPad = MCG_ROW_PAD;
+
 void fill_params(const mcg_case &c, int storage, DevParams &P) {
     P.NX = c.NX;
     P.NY = c.NY;
@@ -173,7 +180,7 @@ void fill_params(const mcg_case &c, int storage, DevParams &P) {
     P.IY = c.NY - 2;
     P.IZ = c.NZ - 2;
     P.plane = uint32_t(size_t(P.PX) * P.IY * P.IZ);
-    P.rowstride = uint32_t(P.PX * (storage == MCG_STORE_FP32 ? 2 * Q : Q));
+    P.rowstride = uint32_t(P.PX * (storage == MCG_STORE_FP32 ? 2 * Q : Q) + kRowPad);
     P.y_mode = c.y_mode;
     P.wrap_gradient = c.wrap_gradient;
     P.visc_mode = c.viscosity_mode;
@@ -386,7 +393,8 @@ int mcg_case_validate(const mcg_case *c) {
     if (c->NX > 2048) return fail(MCG_E_INVALID, "NX > 2048 (radial bins NX/2 must fit the LDS histogram)");
     const size_t px = size_t((c->NX - 2 + 63) / 64 * 64);
     const size_t plane = px * size_t(c->NY - 2) * size_t(c->NZ - 2);
-    if (plane * 54 >= (size_t(1) << 32)) return fail(MCG_E_INVALID, "grid too large for 32-bit population offsets");
+    if ((px * 54 + kRowPad) * size_t(c->NY - 2) * size_t(c->NZ - 2) >= (size_t(1) << 32))
+        return fail(MCG_E_INVALID, "grid too large for 32-bit population offsets");
     if (c->y_mode != MCG_Y_JET && c->y_mode != MCG_Y_PERIODIC) return fail(MCG_E_INVALID, "unknown y_mode");
     if (c->viscosity_mode != MCG_VISC_SINGLE && c->viscosity_mode != MCG_VISC_INTERFACE)
         return fail(MCG_E_INVALID, "unknown viscosity_mode");

@@ -184,13 +184,25 @@ struct FusedGrid {
 #ifndef MCG_FUSED_WAVES_PER_EU
 #define MCG_FUSED_WAVES_PER_EU 3
 #endif
-// core rows per workgroup of the fused kernel
+// Wave roles.  Balanced layout (default): 8 core rows and 4 halo waves, the
+// halo waves being the waves w with w % 3 == 2, so that each of the 4 SIMDs of
+// the CU holds 2 core waves and 1 halo wave whether a workgroup's waves are
+// dealt to the SIMDs round-robin or in blocks of three (the step is VALU-bound
+// on the busiest SIMD with 16-bit storage).  Otherwise: MCG_FUSED_TY core waves
+// followed by 3 halo waves.
+#ifndef MCG_FUSED_BALANCED
+#define MCG_FUSED_BALANCED 1
+#endif
+#if MCG_FUSED_BALANCED
+constexpr int kFusedTY = 8;
+constexpr int kFusedWaves = 12;
+#else
 #ifndef MCG_FUSED_TY
 #define MCG_FUSED_TY 9
 #endif
 constexpr int kFusedTY = MCG_FUSED_TY;
-// TY core waves, two waves for the y-halo rows, one for the x-halo columns
 constexpr int kFusedWaves = kFusedTY + 3;
+#endif
 
 // Tile of workgroup b when the hardware deals workgroups round-robin over the
 // 8 XCDs: XCD k gets a contiguous range of tiles, so tiles that share halo
@@ -200,10 +212,10 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return k * q + min(k, r) + (b >> 3);
 }
 
-// Workgroup = TY core waves (one y-row of 64 x-columns each) + 3 halo waves.
+// Workgroup = TY core waves (one y-row of 64 x-columns each) + halo waves.
 // Every lane owns one gather position of the (TY+2) x 66 psi tile: core lanes
-// their node, halo wave TY the row y0-1, halo wave TY+1 the row y0+TY, halo
-// wave TY+2 (lanes < 2(TY+2)) the columns x0-1 and x0+64.  Iteration zi
+// their node, the first halo wave the row y0-1, the second the row y0+TY, the
+// others the columns x0-1 and x0+64 (lanes < TY+2).  Iteration zi
 // gathers the lead plane zi+1 into the psi ring (core lanes keep the moments
 // in registers), passes one barrier, and collides plane zi; the first two
 // iterations only fill the ring and the moments of plane zb.  The gather and
@@ -229,21 +241,35 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     const int x0 = 1 + xt * 64, y0 = 1 + yb * TY;
     const int zb = 1 + zs * G.zlen;
     const int ze = min(P.NZ - 2, zb + G.zlen - 1);
+#if MCG_FUSED_BALANCED
+    const bool core = (wv % 3) != 2;
+    const int row = core ? wv - wv / 3 : 0;  // core row 0..7
+    const int hrole = core ? -1 : wv / 3;    // halo role 0..3
+#else
     const bool core = wv < TY;
-    const int x = x0 + tx, y = y0 + wv;
+    const int row = core ? wv : 0;
+    const int hrole = core ? -1 : wv - TY;   // halo role 0..2
+#endif
+    const int x = x0 + tx, y = y0 + row;
     const bool node = core && x <= P.NX - 2 && y <= P.NY - 2;
 
     // this lane's gather position in tile coordinates
-    int lr = wv + 1, lc = tx + 1;
+    int lr = row + 1, lc = tx + 1;
     bool active = true;
-    if (wv == TY) {
+    if (hrole == 0) {
         lr = 0;
-    } else if (wv == TY + 1) {
+    } else if (hrole == 1) {
         lr = H - 1;
-    } else if (wv == TY + 2) {
+    } else if (hrole >= 2) {
+#if MCG_FUSED_BALANCED
+        active = tx < H;  // role 2: column x0-1, role 3: column x0+64
+        lr = tx;
+        lc = hrole == 2 ? 0 : W - 1;
+#else
         active = tx < 2 * H;
         lr = tx >> 1;
         lc = (tx & 1) ? W - 1 : 0;
+#endif
     }
     const int gx = x0 - 1 + lc, gy = y0 - 1 + lr;
 
@@ -279,7 +305,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
             const Moments &m = cur;
             auto phi = [&](auto Ic) -> float {
                 constexpr int I = decltype(Ic)::value;
-                return phs[(z + kCZ[I]) & 3][wv + 1 + kCY[I]][tx + 1 + kCX[I]];
+                return phs[(z + kCZ[I]) & 3][row + 1 + kCY[I]][tx + 1 + kCX[I]];
             };
             const NodeColl n = node_setup(P, m, y, phi);
             const DevParams PL = fresh_strides(P);
@@ -298,7 +324,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
                     float ur = 0.0f;
                     if (rr > 1.0e-12f) ur = (m.ux * dx + m.uz * dz) / rr;
                     const double vy = double(m.uy), vr = double(ur), ph = double(psi(m.rr, m.rb));
-                    double *hb = hist + (size_t(wv) * P.NR + bin) * 8;
+                    double *hb = hist + (size_t(row) * P.NR + bin) * 8;
                     atomicAdd(hb + 0, vy);
                     atomicAdd(hb + 1, vy * vy);
                     atomicAdd(hb + 2, vr);

@@ -1,0 +1,130 @@
+"""Offline MLUPS / roofline reporter over rocprofv3 CSV output.
+
+Reads the files ``rocprofv3 --output-format csv`` writes:
+  * ``*_kernel_stats.csv`` (``--kernel-trace --stats``): per-kernel call
+    count and durations;
+  * ``*_counter_collection.csv`` (``--pmc FETCH_SIZE`` or ``--pmc WRITE_SIZE``):
+    one row per dispatch and counter, values in KiB.
+and turns them into the figures the benchmark reports: bytes per launch
+(scaled by a plain copy of known size, see ``copy_probe_scale``), bytes per
+lattice update, GB/s and the fraction of the HBM peak.  MLUPS follows
+utilities/countmlups.cuh:42-48 (steps * NX*NY*NZ / t / 1e6).
+
+Command line:
+  python -m multicomponent_color_gradient_amd.perf stats <kernel_stats.csv>
+  python -m multicomponent_color_gradient_amd.perf traffic <fetch.csv> <write.csv> --kernel k_fused --nodes N
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import sys
+from collections import defaultdict
+
+HBM_PEAK_GBS = 8000.0
+COPY_PROBE_BYTES = 1 << 30
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def mlups(steps: int, cells: int, seconds: float) -> float:
+    """Million lattice updates per second (utilities/countmlups.cuh:42-48)."""
+    return steps * cells / seconds / 1e6
+
+
+def roofline(bytes_per_update: float, updates: int, seconds: float, peak_gbs: float = HBM_PEAK_GBS) -> dict:
+    """Achieved GB/s of bytes_per_update x updates in seconds, and its fraction of peak_gbs."""
+    gbs = bytes_per_update * updates / seconds / 1e9
+    return {"achieved": gbs, "peak": peak_gbs, "unit": "GB/s", "frac": gbs / peak_gbs}
+
+
+def read_kernel_stats(path: str) -> list[dict]:
+    """Rows of a kernel_stats.csv: name, calls, total/avg/min/max ns, percentage."""
+    out = []
+    with open(path, newline="") as fh:
+        for r in csv.DictReader(fh):
+            out.append({"name": r["Name"], "calls": int(r["Calls"]), "total_ns": float(r["TotalDurationNs"]),
+                        "avg_ns": float(r["AverageNs"]), "min_ns": float(r["MinNs"]), "max_ns": float(r["MaxNs"]),
+                        "percent": float(r["Percentage"])})
+    return out
+
+
+def read_counters(path: str) -> dict:
+    """{kernel name: {dispatch id: {counter: value summed over the rows of that dispatch}}}."""
+    out: dict = defaultdict(lambda: defaultdict(lambda: defaultdict(float)))
+    with open(path, newline="") as fh:
+        for r in csv.DictReader(fh):
+            out[r["Kernel_Name"]][int(r["Dispatch_Id"])][r["Counter_Name"]] += float(r["Counter_Value"])
+    return out
+
+
+def per_dispatch(counters: dict, kernel: str, counter: str) -> list[float]:
+    """Values of one counter for every dispatch of the kernels whose name contains ``kernel``, in dispatch order."""
+    vals = []
+    for name, disp in counters.items():
+        if kernel in name:
+            vals += [(d, c[counter]) for d, c in disp.items() if counter in c]
+    return [v for _, v in sorted(vals)]
+
+
+def copy_probe_scale(counters: dict, counter: str, nbytes: int = COPY_PROBE_BYTES, min_kib: float = 1e5) -> float:
+    """Bytes per counter unit, from the probe's device copies of ``nbytes``.
+
+    The probe (``bench.py --copy-probe-mib``) runs three identical copies; they
+    are the first three copy dispatches whose counter exceeds ``min_kib``
+    (smaller copies before them belong to the solver's set-up, larger ones
+    after them to the adjoint).  A copy reads and writes exactly nbytes, so
+    the scale is nbytes / counter."""
+    vals = [v for v in per_dispatch(counters, COPY_KERNEL, counter) if v > min_kib][:3]
+    if len(vals) != 3 or max(vals) - min(vals) > 1e-3 * max(vals):
+        raise ValueError(f"copy probe not found in the {counter} dispatches: {vals}")
+    return nbytes / (sum(vals) / 3)
+
+
+def traffic(fetch: dict, write: dict, kernel: str, nodes: int | None = None) -> dict:
+    """Scaled HBM bytes per launch of ``kernel`` from a FETCH_SIZE and a WRITE_SIZE collection."""
+    sf = copy_probe_scale(fetch, "FETCH_SIZE")
+    sw = copy_probe_scale(write, "WRITE_SIZE")
+    f = per_dispatch(fetch, kernel, "FETCH_SIZE")
+    w = per_dispatch(write, kernel, "WRITE_SIZE")
+    if not f or not w:
+        raise ValueError(f"no dispatches of {kernel}")
+    rd = sum(f) / len(f) * sf
+    wr = sum(w) / len(w) * sw
+    out = {"kernel": kernel, "dispatches": len(f), "read_bytes": rd, "write_bytes": wr, "bytes_per_launch": rd + wr,
+           "fetch_scale": sf * 1 / 1024, "write_scale": sw * 1 / 1024}
+    if nodes:
+        out["read_per_node"] = rd / nodes
+        out["write_per_node"] = wr / nodes
+    return out
+
+
+def stats_table(rows: list[dict]) -> str:
+    lines = ["| kernel | calls | avg us | min us | max us | % |", "|---|---|---|---|---|---|"]
+    for r in rows:
+        lines.append(f"| `{r['name'][:70]}` | {r['calls']} | {r['avg_ns'] / 1e3:.1f} | {r['min_ns'] / 1e3:.1f} | "
+                     f"{r['max_ns'] / 1e3:.1f} | {r['percent']:.2f} |")
+    return "\n".join(lines)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m multicomponent_color_gradient_amd.perf")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("csv")
+    b = sub.add_parser("traffic")
+    b.add_argument("fetch_csv")
+    b.add_argument("write_csv")
+    b.add_argument("--kernel", required=True)
+    b.add_argument("--nodes", type=int, default=None)
+    args = ap.parse_args(argv)
+    if args.cmd == "stats":
+        print(stats_table(read_kernel_stats(args.csv)))
+    else:
+        t = traffic(read_counters(args.fetch_csv), read_counters(args.write_csv), args.kernel, args.nodes)
+        for k, v in t.items():
+            print(f"{k}: {v}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

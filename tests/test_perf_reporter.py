@@ -1,0 +1,85 @@
+"""The offline MLUPS / roofline reporter on synthetic rocprofv3 CSV files."""
+import csv
+
+import pytest
+
+from multicomponent_color_gradient_amd import perf
+
+COUNTER_COLS = ["Correlation_Id", "Dispatch_Id", "Agent_Id", "Queue_Id", "Process_Id", "Thread_Id", "Grid_Size",
+                "Kernel_Id", "Kernel_Name", "Workgroup_Size", "LDS_Block_Size", "Scratch_Size", "VGPR_Count",
+                "Accum_VGPR_Count", "SGPR_Count", "Counter_Name", "Counter_Value", "Start_Timestamp", "End_Timestamp"]
+
+
+def write_counters(path, rows):
+    with open(path, "w", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow(COUNTER_COLS)
+        for disp, name, counter, value in rows:
+            w.writerow([disp, disp, "Agent 1", 1, 1, 1, 1024, 1, name, 256, 0, 0, 32, 0, 32, counter, value, 0, 1])
+
+
+def synthetic(tmp_path):
+    # setup copy (small), probe copies (1 GiB read back as 0.5 GiB FETCH / 1 GiB WRITE),
+    # two dispatches of the kernel, a later big copy (adjoint) that must be ignored
+    gib_kib = (1 << 30) / 1024
+    fetch = [(1, "__amd_rocclr_copyBuffer", "FETCH_SIZE", 12.0),
+             (2, "void mcg::k_fused<false, 0>(...)", "FETCH_SIZE", 3.0e6),
+             (3, "void mcg::k_fused<false, 0>(...)", "FETCH_SIZE", 3.2e6),
+             (4, "__amd_rocclr_copyBuffer", "FETCH_SIZE", gib_kib / 2),
+             (5, "__amd_rocclr_copyBuffer", "FETCH_SIZE", gib_kib / 2),
+             (6, "__amd_rocclr_copyBuffer", "FETCH_SIZE", gib_kib / 2),
+             (7, "__amd_rocclr_copyBuffer", "FETCH_SIZE", 3 * gib_kib)]
+    write = [(d, n, "WRITE_SIZE", v * (2.0 if "copy" in n and v > 1e5 else 1.0)) for d, n, _, v in fetch]
+    write[1] = (2, "void mcg::k_fused<false, 0>(...)", "WRITE_SIZE", 7.0e6)
+    write[2] = (3, "void mcg::k_fused<false, 0>(...)", "WRITE_SIZE", 7.0e6)
+    fp, wp = tmp_path / "fetch.csv", tmp_path / "write.csv"
+    write_counters(fp, fetch)
+    write_counters(wp, write)
+    return str(fp), str(wp)
+
+
+def test_copy_probe_scale_and_traffic(tmp_path):
+    fp, wp = synthetic(tmp_path)
+    f, w = perf.read_counters(fp), perf.read_counters(wp)
+    assert perf.copy_probe_scale(f, "FETCH_SIZE") == pytest.approx(2048.0)  # bytes per KiB unit: counter reads half
+    assert perf.copy_probe_scale(w, "WRITE_SIZE") == pytest.approx(1024.0)
+    t = perf.traffic(f, w, "k_fused", nodes=1000)
+    assert t["dispatches"] == 2
+    assert t["read_bytes"] == pytest.approx(3.1e6 * 2048)
+    assert t["write_bytes"] == pytest.approx(7.0e6 * 1024)
+    assert t["bytes_per_launch"] == pytest.approx(t["read_bytes"] + t["write_bytes"])
+    assert t["read_per_node"] == pytest.approx(t["read_bytes"] / 1000)
+    assert t["fetch_scale"] == pytest.approx(2.0) and t["write_scale"] == pytest.approx(1.0)
+
+
+def test_missing_probe_is_an_error(tmp_path):
+    p = tmp_path / "c.csv"
+    write_counters(p, [(1, "void k(...)", "FETCH_SIZE", 5.0)])
+    with pytest.raises(ValueError):
+        perf.copy_probe_scale(perf.read_counters(str(p)), "FETCH_SIZE")
+
+
+def test_kernel_stats_table(tmp_path):
+    p = tmp_path / "run_kernel_stats.csv"
+    with open(p, "w", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow(["Name", "Calls", "TotalDurationNs", "AverageNs", "Percentage", "MinNs", "MaxNs", "StdDev"])
+        w.writerow(["void mcg::k_fused<false, 0>(mcg::DevParams)", 22, 89953944, 4088815.6, 82.49, 4030021, 4147606, 1])
+    rows = perf.read_kernel_stats(str(p))
+    assert rows[0]["calls"] == 22 and rows[0]["avg_ns"] == pytest.approx(4088815.6)
+    table = perf.stats_table(rows)
+    assert "| 22 | 4088.8 | 4030.0 | 4147.6 | 82.49 |" in table
+
+
+def test_mlups_and_roofline():
+    # the reference's definition: steps * cells / t; 432 B per update at 4.1 ms for 32.9 M updates
+    assert perf.mlups(200, 256 * 512 * 256, 0.8) == pytest.approx(200 * 33554432 / 0.8 / 1e6)
+    r = perf.roofline(432, 254 * 510 * 254, 4.1e-3)
+    assert r["achieved"] == pytest.approx(432 * 254 * 510 * 254 / 4.1e-3 / 1e9)
+    assert r["frac"] == pytest.approx(r["achieved"] / 8000.0)
+
+
+def test_cli(tmp_path, capsys):
+    fp, wp = synthetic(tmp_path)
+    assert perf.main(["traffic", fp, wp, "--kernel", "k_fused", "--nodes", "10"]) == 0
+    assert "bytes_per_launch" in capsys.readouterr().out
