@@ -184,14 +184,13 @@ struct FusedGrid {
 #ifndef MCG_FUSED_WAVES_PER_EU
 #define MCG_FUSED_WAVES_PER_EU 3
 #endif
-// Wave roles.  Balanced layout (default): 8 core rows and 4 halo waves, the
-// halo waves being the waves w with w % 3 == 2, so that each of the 4 SIMDs of
-// the CU holds 2 core waves and 1 halo wave whether a workgroup's waves are
-// dealt to the SIMDs round-robin or in blocks of three (the step is VALU-bound
-// on the busiest SIMD with 16-bit storage).  Otherwise: MCG_FUSED_TY core waves
-// followed by 3 halo waves.
+// Wave roles.  Default: MCG_FUSED_TY = 9 core waves followed by 3 halo waves
+// (228 workgroups at 256x512x256).  MCG_FUSED_BALANCED=1: 8 core rows and 4
+// halo waves (the waves w with w % 3 == 2), which gives every SIMD 2 core
+// waves and 1 halo wave and 256 workgroups at 256x512x256; measured 6% (FP32)
+// and 24% (FP16) slower than the default on MI355X, so it is off.
 #ifndef MCG_FUSED_BALANCED
-#define MCG_FUSED_BALANCED 1
+#define MCG_FUSED_BALANCED 0
 #endif
 #if MCG_FUSED_BALANCED
 constexpr int kFusedTY = 8;
