@@ -240,6 +240,16 @@ struct Store16 {
     __device__ __forceinline__ static void store(void *f, uint32_t stride, uint32_t off, float fr, float fb) {
         __builtin_nontemporal_store(encode<I>(fr, fb), &static_cast<uint32_t *>(f)[uint32_t(I) * stride + off]);
     }
+    // word from the offset a = t/w - 1 and the colours already divided by w
+    __device__ __forceinline__ static uint32_t encode_w(float a, float rw, float bw) {
+        const bool red_major = rw >= bw;
+        const float m = fabsf(red_major ? bw : rw);
+        return half_bits(a) | ((half_bits(m) | (red_major ? 0u : 0x8000u)) << 16);
+    }
+    template <int I>
+    __device__ __forceinline__ static void store_w(void *f, uint32_t stride, uint32_t off, float a, float rw, float bw) {
+        __builtin_nontemporal_store(encode_w(a, rw, bw), &static_cast<uint32_t *>(f)[uint32_t(I) * stride + off]);
+    }
 };
 template <>
 struct Store<MCG_STORE_FP16> : Store16<MCG_STORE_FP16> {};
@@ -408,6 +418,31 @@ __device__ __forceinline__ void collide_node(const NodeColl &n, PutFn &&put) {
         put(std::integral_constant<int, I>{}, rp, bp);
         put(std::integral_constant<int, I + 1>{}, rm, bm);
     });
+}
+
+// Collide node n and store its 54 post-collision populations at element p of
+// the row block (FP32 rows, or 16-bit words built directly from the w-scaled
+// terms).
+template <int ST>
+__device__ __forceinline__ void collide_store(const NodeColl &n, void *f, uint32_t PX, uint32_t p) {
+    if constexpr (ST == MCG_STORE_FP32) {
+        collide_node(n, [&](auto Ic, float a, float b) {
+            Store<ST>::template store<decltype(Ic)::value>(f, PX, p, a, b);
+        });
+    } else {
+        {
+            float E = fma_(n.q2, n.tr, n.e0);
+            E = fma_(n.sB, 1.25f, E);
+            Store<ST>::template store_w<0>(f, PX, p, E - 1.0f, n.aR * E, n.aB * E);
+        }
+        unroll<0, 13>([&](auto kc) {
+            constexpr int I = 2 * decltype(kc)::value + 1;
+            float ap, rp, bp, am, rm, bm;
+            collide_pair_w<I>(n, ap, rp, bp, am, rm, bm);
+            Store<ST>::template store_w<I>(f, PX, p, ap, rp, bp);
+            Store<ST>::template store_w<I + 1>(f, PX, p, am, rm, bm);
+        });
+    }
 }
 
 // SplitMix64 (Steele, Lea and Flood 2014, the finaliser constants of Vigna's

@@ -309,9 +309,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
             const NodeColl n = node_setup(P, m, y, phi);
             const DevParams PL = fresh_strides(P);
             const uint32_t p = pidx(PL, x, y, z);
-            collide_node(n, [&](auto Ic, float a, float b) {
-                Store<ST>::template store<decltype(Ic)::value>(B.f_dst, uint32_t(PL.PX), p, a, b);
-            });
+            collide_store<ST>(n, B.f_dst, uint32_t(PL.PX), p);
             if (P.y_mode == MCG_Y_JET) {
                 if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z, B.step);
                 if (y == P.NY - 2) outlet_emit(P, B.gB_src, B.gB_dst, B.out_dst, m, x, z);

@@ -155,9 +155,7 @@ __global__ __launch_bounds__(256) void k_collide(DevParams P, DevBufs B) {
         return ghost_psi(P, B, xn, yn, zn);
     };
     const NodeColl n = node_setup(P, m, y, phi);
-    collide_node(n, [&](auto Ic, float a, float b) {
-        Store<ST>::template store<decltype(Ic)::value>(B.f_dst, uint32_t(P.PX), p, a, b);
-    });
+    collide_store<ST>(n, B.f_dst, uint32_t(P.PX), p);
     if (P.y_mode == MCG_Y_JET) {
         if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z, B.step);
         if (y == P.NY - 2) outlet_emit(P, B.gB_src, B.gB_dst, B.out_dst, m, x, z);

@@ -178,9 +178,10 @@ __device__ __forceinline__ void collide_rest(const NodeColl &n, float &r, float 
     b = n.aB * g;
 }
 
-// direction I and its opposite I+1 (directions 2k-1 and 2k are opposite)
+// Even and odd parts of the pair (I, I+1) divided by w_I: E, O and the
+// recolouring term D (directions 2k-1 and 2k are opposite).
 template <int I>
-__device__ __forceinline__ void collide_pair(const NodeColl &n, float &rp, float &bp, float &rm, float &bm) {
+__device__ __forceinline__ void pair_terms(const NodeColl &n, float &E, float &O, float &D) {
     static_assert(I % 2 == 1 && kCX[I] == -kCX[I + 1] && kCY[I] == -kCY[I + 1] && kCZ[I] == -kCZ[I + 1], "pair");
     const float cu = cdot<I>(n.ux, n.uy, n.uz);
     const float cu2 = cu * cu;
@@ -200,13 +201,20 @@ __device__ __forceinline__ void collide_pair(const NodeColl &n, float &rp, float
     if constexpr (Hyzz(I) != 0.0f) a3 = fma_(n.tyzz, Hyzz(I), a3);
     if constexpr (Hxyz(I) != 0.0f) a3 = fma_(n.txyz, Hxyz(I), a3);
     const float d = cdot<I>(n.Fx, n.Fy, n.Fz);
-    float E = fma_(n.e1, cu2, n.e0);
+    E = fma_(n.e1, cu2, n.e0);
     E = fma_(n.q2, a2, E);
     E = fma_(n.s2, d * d, E) - n.sB;
-    float O = cu * fma_(n.o3, cu2, n.o1);
+    O = cu * fma_(n.o3, cu2, n.o1);
     O = fma_(n.q3, a3, O);
-    float D = n.rc * d;
+    D = n.rc * d;
     if constexpr (csq(I) > 1) D = D * invc(I);
+}
+
+// direction I and its opposite I+1
+template <int I>
+__device__ __forceinline__ void collide_pair(const NodeColl &n, float &rp, float &bp, float &rm, float &bm) {
+    float E, O, D;
+    pair_terms<I>(n, E, O, D);
     const float gp = w(I) * (E + O);
     const float gm = w(I) * (E - O);
     const float Dw = w(I) * D;
@@ -214,6 +222,23 @@ __device__ __forceinline__ void collide_pair(const NodeColl &n, float &rp, float
     bp = fma_(n.aB, gp, -Dw);
     rm = fma_(n.aR, gm, -Dw);
     bm = fma_(n.aB, gm, Dw);
+}
+
+// The same pair for 16-bit storage, which keeps populations divided by w_I:
+// the offsets (E +- O) - 1 and the two colours (E +- O) aR +- D, (E +- O) aB -+ D.
+template <int I>
+__device__ __forceinline__ void collide_pair_w(const NodeColl &n, float &ap, float &rp, float &bp, float &am, float &rm,
+                                               float &bm) {
+    float E, O, D;
+    pair_terms<I>(n, E, O, D);
+    const float gp = E + O, gm = E - O;
+    const float em1 = E - 1.0f;
+    ap = em1 + O;
+    am = em1 - O;
+    rp = fma_(n.aR, gp, D);
+    bp = fma_(n.aB, gp, -D);
+    rm = fma_(n.aR, gm, -D);
+    bm = fma_(n.aB, gm, D);
 }
 
 // Regularised boundary population (inlet jet/boundary_conditions.cuh:128-147,
