@@ -422,6 +422,10 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
     s->storage = storage;
     s->stream = static_cast<hipStream_t>(stream);
     fill_params(*c, storage, s->P);
+    if (storage != MCG_STORE_FP32 && s->pop_bytes() >= (size_t(1) << 32)) {
+        delete s;
+        return fail(MCG_E_INVALID, "16-bit population storage is limited to 4 GiB per buffer (32-bit byte offsets)");
+    }
     if (hipGetDevice(&s->device) != hipSuccess) {
         delete s;
         return fail(MCG_E_HIP, "hipGetDevice failed (no GPU?)");

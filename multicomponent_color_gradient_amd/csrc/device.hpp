@@ -187,6 +187,15 @@ struct Store<MCG_STORE_FP32> {
 template <int ST>
 struct Store16 {
     static constexpr int planes = Q;
+    // word idx of the population array as a 32-bit byte offset from its base:
+    // the arrays of 16-bit storage are limited to 4 GiB, so the loads and
+    // stores address base (scalar) + offset (32-bit vector register)
+    __device__ __forceinline__ static const uint32_t *at(const void *f, uint32_t idx) {
+        return reinterpret_cast<const uint32_t *>(static_cast<const char *>(f) + (idx << 2));
+    }
+    __device__ __forceinline__ static uint32_t *at(void *f, uint32_t idx) {
+        return reinterpret_cast<uint32_t *>(static_cast<char *>(f) + (idx << 2));
+    }
     __device__ __forceinline__ static uint32_t half_bits(float v) {
         if constexpr (ST == MCG_STORE_FP16) {
             return uint32_t(__builtin_bit_cast(uint16_t, static_cast<_Float16>(v)));
@@ -225,12 +234,12 @@ struct Store16 {
     }
     template <int I>
     __device__ __forceinline__ static void load(const void *f, uint32_t stride, uint32_t off, float &fr, float &fb) {
-        decode<I>(static_cast<const uint32_t *>(f)[uint32_t(I) * stride + off], fr, fb);
+        decode<I>(*at(f, uint32_t(I) * stride + off), fr, fb);
     }
     static constexpr int kWords = 1;
     template <int I>
     __device__ __forceinline__ static void load_raw(const void *f, uint32_t stride, uint32_t off, uint32_t *w) {
-        w[0] = static_cast<const uint32_t *>(f)[uint32_t(I) * stride + off];
+        w[0] = *at(f, uint32_t(I) * stride + off);
     }
     template <int I>
     __device__ __forceinline__ static void decode_raw(const uint32_t *w, float &fr, float &fb) {
@@ -238,7 +247,7 @@ struct Store16 {
     }
     template <int I>
     __device__ __forceinline__ static void store(void *f, uint32_t stride, uint32_t off, float fr, float fb) {
-        __builtin_nontemporal_store(encode<I>(fr, fb), &static_cast<uint32_t *>(f)[uint32_t(I) * stride + off]);
+        __builtin_nontemporal_store(encode<I>(fr, fb), at(f, uint32_t(I) * stride + off));
     }
     // word from the offset a = t/w - 1 and the colours already divided by w
     __device__ __forceinline__ static uint32_t encode_w(float a, float rw, float bw) {
@@ -248,7 +257,7 @@ struct Store16 {
     }
     template <int I>
     __device__ __forceinline__ static void store_w(void *f, uint32_t stride, uint32_t off, float a, float rw, float bw) {
-        __builtin_nontemporal_store(encode_w(a, rw, bw), &static_cast<uint32_t *>(f)[uint32_t(I) * stride + off]);
+        __builtin_nontemporal_store(encode_w(a, rw, bw), at(f, uint32_t(I) * stride + off));
     }
 };
 template <>
