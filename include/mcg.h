@@ -247,6 +247,22 @@ int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const doub
                     const double *lam_fb, double *out_fr, double *out_fb, double *out_lam_fr, double *out_lam_fb,
                     double *dparams);
 
+/* The FP64 CPU forward and adjoint for jet cases (y_mode jet, reference
+ * colour gradient, single relaxation rate): the inlet and the convective
+ * outlet are differentiated, the outlet state (rho_r, rho_b, u on the plane
+ * NY-1) is carried between steps.  fields0 = rho_r, rho_b, u_x, u_y, u_z of the
+ * initial state over the full grid (5 x NX*NY*NZ, ghosts included: they give
+ * the constant ghost psi and the initial outlet state).  J = <lam, f_final> +
+ * sum over the moments passes of steps uy_first..uy_last of sum_x w_uy(x) u_y(x)
+ * (w_uy may be NULL).  dparams = {dJ/dA, dJ/dbeta, dJ/djet_velocity,
+ * dJ/dperturbation_amp}; dinlet = dJ/de for the inlet velocity e of each jet
+ * column summed over steps (3 x NX*NZ, component-major).  No counterpart in the
+ * reference (an extension). */
+int mcg_cpu_adjoint_jet(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *fields0,
+                        const double *lam_fr, const double *lam_fb, const double *w_uy, int uy_first, int uy_last,
+                        double *out_fr, double *out_fb, double *out_lam_fr, double *out_lam_fb, double *dparams,
+                        double *dinlet);
+
 /* ---- measurement ------------------------------------------------------ */
 
 /* Run nsteps steps bracketed by HIP events on the solver's stream and return

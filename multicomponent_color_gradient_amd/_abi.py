@@ -139,6 +139,7 @@ SIGNATURES = [
     ("mcg_adjoint_last_ms", c_int, [_PSOLVER, _D]),
     ("mcg_adjoint_release", c_int, [_PSOLVER]),
     ("mcg_cpu_adjoint", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
+    ("mcg_cpu_adjoint_jet", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, c_int, c_int, _D, _D, _D, _D, _D, _D]),
     ("mcg_time_steps", c_int, [_PSOLVER, c_int, _D]),
     ("mcg_device_bytes", c_int, [_PSOLVER, POINTER(c_uint64)]),
     ("mcg_output_dirs", c_int, [POINTER(McgCase), c_char_p, c_char_p, c_char_p, c_char_p, c_size_t]),

@@ -46,6 +46,46 @@ def cpu_adjoint(case: McgCase, fr0: np.ndarray, fb0: np.ndarray, nsteps: int, la
     return out
 
 
+def cpu_adjoint_jet(case: McgCase, fr0: np.ndarray, fb0: np.ndarray, fields0: dict, nsteps: int,
+                    lam_fr: np.ndarray | None = None, lam_fb: np.ndarray | None = None, w_uy: np.ndarray | None = None,
+                    uy_steps: tuple[int, int] = (0, -1)) -> dict:
+    """FP64 forward and adjoint of a jet case (inlet and convective outlet
+    differentiated).  fields0: the initial fields dict (rhor, rhob, ux, uy, uz
+    over the full grid, ghosts included).  Returns the final state and, if a
+    seed is given, {"lam_fr", "lam_fb", "dA", "dbeta", "dU", "damp", "dinlet"}
+    (dinlet: dJ/de per jet column summed over steps, shape (3, NZ, NX)).  J is
+    <lam, f_final> plus sum over the moments passes of steps uy_steps of
+    sum_x w_uy(x) u_y(x)."""
+    lib = load_library()
+    shape = (27, case.NZ, case.NY, case.NX)
+    d = ctypes.c_double
+    fr0 = np.ascontiguousarray(fr0, np.float64).reshape(shape)
+    fb0 = np.ascontiguousarray(fb0, np.float64).reshape(shape)
+    f0 = np.ascontiguousarray(np.stack([fields0[k] for k in ("rhor", "rhob", "ux", "uy", "uz")]), np.float64)
+    out = {"fr": np.empty(shape), "fb": np.empty(shape)}
+    seeded = lam_fr is not None or w_uy is not None
+    if lam_fr is None:
+        lam_fr = np.zeros(shape)
+        lam_fb = np.zeros(shape)
+    lam_fr = np.ascontiguousarray(lam_fr, np.float64).reshape(shape)
+    lam_fb = np.ascontiguousarray(lam_fb, np.float64).reshape(shape)
+    w = None if w_uy is None else np.ascontiguousarray(w_uy, np.float64).reshape(shape[1:])
+    dp = np.zeros(4)
+    dinlet = np.zeros((3, case.NZ, case.NX))
+    out["lam_fr"], out["lam_fb"] = np.empty(shape), np.empty(shape)
+    c = case.copy()
+    check(lib.mcg_cpu_adjoint_jet(ctypes.byref(c), int(nsteps), as_ptr(fr0, d), as_ptr(fb0, d), as_ptr(f0, d),
+                                  as_ptr(lam_fr, d) if seeded else None, as_ptr(lam_fb, d) if seeded else None,
+                                  None if w is None else as_ptr(w, d), int(uy_steps[0]), int(uy_steps[1]),
+                                  as_ptr(out["fr"], d), as_ptr(out["fb"], d), as_ptr(out["lam_fr"], d),
+                                  as_ptr(out["lam_fb"], d), as_ptr(dp, d), as_ptr(dinlet, d)))
+    if seeded:
+        out.update(dA=float(dp[0]), dbeta=float(dp[1]), dU=float(dp[2]), damp=float(dp[3]), dinlet=dinlet)
+    else:
+        del out["lam_fr"], out["lam_fb"]
+    return out
+
+
 def _torch():
     import torch
 

@@ -321,4 +321,356 @@ struct CpuAdjoint {
     }
 };
 
+// ---------------------------------------------------------------- CPU FP64, jet
+// Jet cases (y_mode jet, reference gradient without wrap): x/z periodic
+// streaming; plane y=1 receives c_y=+1 populations from the inlet (jet
+// columns) or keeps them (stale slots), plane NY-2 receives c_y=-1
+// populations from the convective outlet (inner columns) or keeps them; the
+// outlet state gB (rho_r, rho_b, u at y=NY-1) is carried from step to step and
+// gives psi of the ghost plane NY-1 to the colour gradient; other ghost psi
+// are constant.  Inlet velocity e(x,z,n) = U profile(r) + amp U (2 xi - 1)
+// (sharp profile: (0, U, 0)).  The adjoint returns dJ/df at the initial state,
+// dJ/dA, dJ/dbeta, dJ/dU, dJ/d(amp) and dJ/de summed over steps per jet column;
+// an optional seed w(x) on u_y at the moments passes of steps first..last adds
+// sum_n sum_x w(x) u_y^n(x) to J.
+inline uint64_t splitmix64_host(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+struct CpuJetAdjoint {
+    mcg_case c{};
+    int NX = 0, NY = 0, NZ = 0;
+    size_t N = 0;
+    double A = 0, beta = 0, omega = 0;
+    std::vector<double> gpsi;   // psi of every position at the initial state (ghosts constant)
+
+    size_t id(int x, int y, int z) const { return size_t(x) + size_t(NX) * (size_t(y) + size_t(NY) * size_t(z)); }
+    static int wrap(int v, int n) { return v == 0 ? n - 2 : (v == n - 1 ? 1 : v); }
+    bool interior(int x, int y, int z) const { return x >= 1 && x <= NX - 2 && y >= 1 && y <= NY - 2 && z >= 1 && z <= NZ - 2; }
+    bool inner(int x, int z) const { return x >= 1 && x <= NX - 2 && z >= 1 && z <= NZ - 2; }
+    bool is_jet(int x, int z) const {
+        const double dx = double(x) - c.jet_x0, dz = double(z) - c.jet_z0;
+        return dx * dx + dz * dz <= c.jet_radius * c.jet_radius;
+    }
+    double xi(int64_t n, int x, int z, int k) const {
+        const uint64_t key = splitmix64_host(splitmix64_host(splitmix64_host(uint64_t(uint32_t(c.perturbation_seed))) ^ uint64_t(n)) ^
+                                             (uint64_t(z) * uint64_t(NX) + uint64_t(x)));
+        return double(splitmix64_host(key + uint64_t(k)) >> 40) / 16777216.0;
+    }
+    // inlet velocity and its derivatives with respect to U and amp
+    void inlet(int64_t n, int x, int z, double e[3], double dU[3], double damp[3]) const {
+        double prof = 1.0;
+        e[0] = e[2] = 0.0;
+        dU[0] = dU[2] = damp[0] = damp[1] = damp[2] = 0.0;
+        if (c.inlet_profile == MCG_INLET_TANH) {
+            const double dx = double(x) - c.jet_x0, dz = double(z) - c.jet_z0;
+            prof = 0.5 * (1.0 - std::tanh((std::sqrt(dx * dx + dz * dz) - c.jet_radius) / c.interface_width));
+            const double sx = 2.0 * xi(n, x, z, 1) - 1.0, sz = 2.0 * xi(n, x, z, 2) - 1.0;
+            e[0] = c.perturbation_amp * c.jet_velocity * sx;
+            e[2] = c.perturbation_amp * c.jet_velocity * sz;
+            dU[0] = c.perturbation_amp * sx;
+            dU[2] = c.perturbation_amp * sz;
+            damp[0] = c.jet_velocity * sx;
+            damp[2] = c.jet_velocity * sz;
+        }
+        e[1] = c.jet_velocity * prof;
+        dU[1] = prof;
+    }
+
+    struct Step {
+        std::vector<double> M, ps;  // interior moments (11 x N), psi of every position
+    };
+
+    void prepare(const double *f_r, const double *f_b, const double *gB, Step &st) const {
+        st.M.assign(11 * N, 0.0);
+        st.ps = gpsi;
+        for (int z = 1; z < NZ - 1; ++z)
+            for (int y = 1; y < NY - 1; ++y)
+                for (int x = 1; x < NX - 1; ++x) {
+                    const size_t g = id(x, y, z);
+                    double sr = 0, sb = 0, j[3] = {0, 0, 0}, S[6] = {0, 0, 0, 0, 0, 0};
+                    for (int i = 0; i < Q; ++i) {
+                        const double a = f_r[size_t(i) * N + g], b = f_b[size_t(i) * N + g], gg = a + b;
+                        sr += a;
+                        sb += b;
+                        j[0] += kCX[i] * gg;
+                        j[1] += kCY[i] * gg;
+                        j[2] += kCZ[i] * gg;
+                        for (int k = 0; k < 6; ++k) S[k] += gg * h2T<double>(k, i);
+                    }
+                    const double rho = sr + sb, u[3] = {j[0] / rho, j[1] / rho, j[2] / rho};
+                    double *m[11];
+                    for (int k = 0; k < 11; ++k) m[k] = &st.M[size_t(k) * N + g];
+                    *m[0] = sr;
+                    *m[1] = sb;
+                    *m[2] = u[0];
+                    *m[3] = u[1];
+                    *m[4] = u[2];
+                    *m[5 + PXX] = S[PXX] - rho * u[0] * u[0];
+                    *m[5 + PXY] = S[PXY] - rho * u[0] * u[1];
+                    *m[5 + PYY] = S[PYY] - rho * u[1] * u[1];
+                    *m[5 + PYZ] = S[PYZ] - rho * u[1] * u[2];
+                    *m[5 + PZZ] = S[PZZ] - rho * u[2] * u[2];
+                    *m[5 + PXZ] = S[PXZ] - rho * u[0] * u[2];
+                    st.ps[g] = (sr - sb) / rho;
+                }
+        // the outlet ghost plane holds the convective state of the previous step
+        for (int z = 1; z < NZ - 1; ++z)
+            for (int x = 1; x < NX - 1; ++x) {
+                const size_t o = size_t(z) * NX + x;
+                const double rr = gB[o], rb = gB[size_t(NX) * NZ + o];
+                st.ps[id(x, NY - 1, z)] = (rr - rb) / (rr + rb);
+            }
+    }
+
+    void mom(const Step &st, size_t g, double m[11]) const {
+        for (int k = 0; k < 11; ++k) m[k] = st.M[size_t(k) * N + g];
+    }
+
+    AdjNode<double> node(const Step &st, int x, int y, int z) const {
+        double m[11], F[3] = {0, 0, 0};
+        mom(st, id(x, y, z), m);
+        for (int i = 1; i < Q; ++i) {
+            const double p = st.ps[id(x + kCX[i], y + kCY[i], z + kCZ[i])];  // no wrap (forces.cuh:31-33)
+            F[0] += 3.0 * wT<double>(i) * p * kCX[i];
+            F[1] += 3.0 * wT<double>(i) * p * kCY[i];
+            F[2] += 3.0 * wT<double>(i) * p * kCZ[i];
+        }
+        return adj_setup<double>(m, F[0], F[1], F[2], A, beta, omega);
+    }
+
+    // boundary populations: inlet (9 c_y=+1 values, blue) of jet column (x,z)
+    void inlet_pops(const Step &st, int64_t n, int x, int z, double b[9]) const {
+        double m[11], e[3], dU[3], da[3];
+        mom(st, id(x, 1, z), m);
+        inlet(n, x, z, e, dU, da);
+        const double oms = 1.0 - 1.0 / c.taub;
+        for (int k = 0; k < 9; ++k) {
+            const int i = kUp[k];
+            b[k] = feqT<double>(i, c.rhob0, e[0], e[1], e[2]) + oms * fneqrT<double>(i, m + 5, m[2], m[3], m[4]);
+        }
+    }
+
+    // outlet of inner column (x,z): new state gB1 and the 9 c_y=-1 values
+    void outlet_pops(const Step &st, const double *gB, int x, int z, double gB1[5], double r[9], double b[9]) const {
+        double m[11];
+        mom(st, id(x, NY - 2, z), m);
+        const size_t o = size_t(z) * NX + x, nxz = size_t(NX) * NZ;
+        const double uc = std::min(std::max(m[3], 0.0), 1.0);
+        const double mf[5] = {m[0], m[1], m[2], m[3], m[4]};
+        for (int k = 0; k < 5; ++k) gB1[k] = gB[k * nxz + o] - uc * (gB[k * nxz + o] - mf[k]);
+        const double rhoT = gB1[0] + gB1[1], aR = gB1[0] / rhoT, aB = gB1[1] / rhoT;
+        const double oms = 1.0 - omega;
+        for (int k = 0; k < 9; ++k) {
+            const int i = kDown[k];
+            const double g = feqT<double>(i, rhoT, gB1[2], gB1[3], gB1[4]) + oms * fneqrT<double>(i, m + 5, m[2], m[3], m[4]);
+            r[k] = aR * g;
+            b[k] = aB * g;
+        }
+    }
+
+    void step(int64_t n, const double *fr, const double *fb, const double *gB, double *gr, double *gb, double *gB_out) const {
+        Step st;
+        prepare(fr, fb, gB, st);
+        const size_t nxz = size_t(NX) * NZ;
+        std::copy(fr, fr + size_t(Q) * N, gr);  // stale slots keep their values
+        std::copy(fb, fb + size_t(Q) * N, gb);
+        for (int z = 1; z < NZ - 1; ++z)
+            for (int y = 1; y < NY - 1; ++y)
+                for (int x = 1; x < NX - 1; ++x) {
+                    const AdjNode<double> nd = node(st, x, y, z);
+                    double r[Q], b[Q];
+                    adj_collide<double>(nd, r, b);
+                    for (int i = 0; i < Q; ++i) {
+                        const int ty = y + kCY[i];
+                        if (ty < 1 || ty > NY - 2) continue;  // leaves through the inlet/outlet planes
+                        const size_t t = id(wrap(x + kCX[i], NX), ty, wrap(z + kCZ[i], NZ));
+                        gr[size_t(i) * N + t] = r[i];
+                        gb[size_t(i) * N + t] = b[i];
+                    }
+                }
+        std::copy(gB, gB + 5 * nxz, gB_out);
+        for (int z = 1; z < NZ - 1; ++z)
+            for (int x = 1; x < NX - 1; ++x) {
+                if (is_jet(x, z)) {
+                    double b[9];
+                    inlet_pops(st, n, x, z, b);
+                    for (int k = 0; k < 9; ++k) {
+                        const int i = kUp[k], tx = x + kCX[i], tz = z + kCZ[i];
+                        if (!inner(tx, tz)) continue;
+                        const size_t t = id(tx, 1, tz);
+                        gr[size_t(i) * N + t] = 0.0 * b[k];
+                        gb[size_t(i) * N + t] = b[k];
+                    }
+                }
+                double gB1[5], r[9], b[9];
+                outlet_pops(st, gB, x, z, gB1, r, b);
+                const size_t o = size_t(z) * NX + x;
+                for (int k = 0; k < 5; ++k) gB_out[k * nxz + o] = gB1[k];
+                for (int k = 0; k < 9; ++k) {
+                    const int i = kDown[k], tx = x + kCX[i], tz = z + kCZ[i];
+                    if (!inner(tx, tz)) continue;
+                    const size_t t = id(tx, NY - 2, tz);
+                    gr[size_t(i) * N + t] = r[k];
+                    gb[size_t(i) * N + t] = b[k];
+                }
+            }
+    }
+
+    // adjoint of step n: (lr, lb, lgB) of the state after the step -> (outr, outb, outgB) before it
+    void adjoint(int64_t n, const double *fr, const double *fb, const double *gB, const double *lr, const double *lb,
+                 const double *lgB, double *outr, double *outb, double *outgB, double *dp, double *dinlet,
+                 const double *w_uy) const {
+        Step st;
+        prepare(fr, fb, gB, st);
+        const size_t nxz = size_t(NX) * NZ;
+        std::vector<double> LM(11 * N, 0.0), LF(3 * N, 0.0);
+        // collision
+        for (int z = 1; z < NZ - 1; ++z)
+            for (int y = 1; y < NY - 1; ++y)
+                for (int x = 1; x < NX - 1; ++x) {
+                    const AdjNode<double> nd = node(st, x, y, z);
+                    double mr[Q], mb[Q];
+                    for (int i = 0; i < Q; ++i) {
+                        const int ty = y + kCY[i];
+                        mr[i] = mb[i] = 0.0;
+                        if (ty < 1 || ty > NY - 2) continue;
+                        const size_t t = id(wrap(x + kCX[i], NX), ty, wrap(z + kCZ[i], NZ));
+                        mr[i] = lr[size_t(i) * N + t];
+                        mb[i] = lb[size_t(i) * N + t];
+                    }
+                    const AdjOut<double> a = adj_vjp<double>(nd, mr, mb);
+                    const size_t g = id(x, y, z);
+                    for (int k = 0; k < 11; ++k) LM[size_t(k) * N + g] = a.m[k];
+                    for (int k = 0; k < 3; ++k) LF[size_t(k) * N + g] = a.F[k];
+                    dp[0] += a.A;
+                    dp[1] += a.beta;
+                    if (w_uy) LM[3 * N + g] += w_uy[g];
+                }
+        std::copy(lgB, lgB + 5 * nxz, outgB);
+        const double oms_in = 1.0 - 1.0 / c.taub, oms_out = 1.0 - omega;
+        for (int z = 1; z < NZ - 1; ++z)
+            for (int x = 1; x < NX - 1; ++x) {
+                const size_t o = size_t(z) * NX + x;
+                // inlet
+                if (is_jet(x, z)) {
+                    double m[11], e[3], dU[3], da[3], le[3] = {0, 0, 0}, lP[6] = {0, 0, 0, 0, 0, 0}, lu[3] = {0, 0, 0};
+                    const size_t g = id(x, 1, z);
+                    mom(st, g, m);
+                    inlet(n, x, z, e, dU, da);
+                    for (int k = 0; k < 9; ++k) {
+                        const int i = kUp[k], tx = x + kCX[i], tz = z + kCZ[i];
+                        if (!inner(tx, tz)) continue;
+                        const double lg = lb[size_t(i) * N + id(tx, 1, tz)];  // f_r = 0 * g
+                        double l_rho = 0.0;
+                        feq_vjp<double>(i, c.rhob0, e[0], e[1], e[2], lg, l_rho, le);
+                        fneqr_vjp<double>(i, m + 5, m[2], m[3], m[4], oms_in * lg, lP, lu);
+                    }
+                    for (int k = 0; k < 6; ++k) LM[size_t(5 + k) * N + g] += lP[k];
+                    for (int k = 0; k < 3; ++k) LM[size_t(2 + k) * N + g] += lu[k];
+                    for (int k = 0; k < 3; ++k) {
+                        dinlet[size_t(k) * nxz + o] += le[k];
+                        dp[2] += le[k] * dU[k];
+                        dp[3] += le[k] * da[k];
+                    }
+                }
+                // outlet
+                {
+                    double m[11], gB1[5], r[9], b[9];
+                    const size_t g = id(x, NY - 2, z);
+                    mom(st, g, m);
+                    outlet_pops(st, gB, x, z, gB1, r, b);
+                    const double rhoT = gB1[0] + gB1[1], aR = gB1[0] / rhoT, aB = gB1[1] / rhoT;
+                    double l_aR = 0, l_aB = 0, l_rhoT = 0, l_uB[3] = {0, 0, 0}, lP[6] = {0, 0, 0, 0, 0, 0}, lu[3] = {0, 0, 0};
+                    for (int k = 0; k < 9; ++k) {
+                        const int i = kDown[k], tx = x + kCX[i], tz = z + kCZ[i];
+                        if (!inner(tx, tz)) continue;
+                        const size_t t = id(tx, NY - 2, tz);
+                        const double lrk = lr[size_t(i) * N + t], lbk = lb[size_t(i) * N + t];
+                        const double gk = aB > 0 || aR > 0 ? (r[k] + b[k]) : 0.0;  // g = (aR + aB) g
+                        l_aR += gk * lrk;
+                        l_aB += gk * lbk;
+                        const double lg = aR * lrk + aB * lbk;
+                        feq_vjp<double>(i, rhoT, gB1[2], gB1[3], gB1[4], lg, l_rhoT, l_uB);
+                        fneqr_vjp<double>(i, m + 5, m[2], m[3], m[4], oms_out * lg, lP, lu);
+                    }
+                    double L[5];
+                    const double q = (gB1[0] * l_aR + gB1[1] * l_aB) / (rhoT * rhoT);
+                    L[0] = l_aR / rhoT + l_rhoT - q + lgB[o];
+                    L[1] = l_aB / rhoT + l_rhoT - q + lgB[nxz + o];
+                    for (int k = 0; k < 3; ++k) L[2 + k] = l_uB[k] + lgB[(2 + k) * nxz + o];
+                    const double uy = m[3], uc = std::min(std::max(uy, 0.0), 1.0);
+                    const double mf[5] = {m[0], m[1], m[2], m[3], m[4]};
+                    double l_uc = 0.0;
+                    for (int k = 0; k < 5; ++k) {
+                        const double gBk = gB[k * nxz + o];
+                        outgB[k * nxz + o] = (1.0 - uc) * L[k];
+                        LM[size_t(k) * N + g] += uc * L[k];
+                        l_uc += -(gBk - mf[k]) * L[k];
+                    }
+                    if (uy > 0.0 && uy < 1.0) LM[3 * N + g] += l_uc;
+                    for (int k = 0; k < 6; ++k) LM[size_t(5 + k) * N + g] += lP[k];
+                    for (int k = 0; k < 3; ++k) LM[size_t(2 + k) * N + g] += lu[k];
+                }
+            }
+        // colour gradient transpose: psi of interior nodes and of the outlet ghost plane
+        auto Lpsi = [&](int x, int y, int z) {
+            double s = 0.0;
+            for (int i = 1; i < Q; ++i) {
+                const int sx = x - kCX[i], sy = y - kCY[i], sz = z - kCZ[i];
+                if (!interior(sx, sy, sz)) continue;
+                const size_t q = id(sx, sy, sz);
+                s += 3.0 * wT<double>(i) * (kCX[i] * LF[q] + kCY[i] * LF[N + q] + kCZ[i] * LF[2 * N + q]);
+            }
+            return s;
+        };
+        for (int z = 1; z < NZ - 1; ++z)
+            for (int x = 1; x < NX - 1; ++x) {
+                const size_t o = size_t(z) * NX + x;
+                const double lp = Lpsi(x, NY - 1, z);
+                const double rr = gB[o], rb = gB[nxz + o], rho = rr + rb;
+                outgB[o] += lp * 2.0 * rb / (rho * rho);
+                outgB[nxz + o] += -lp * 2.0 * rr / (rho * rho);
+            }
+        std::fill(outr, outr + size_t(Q) * N, 0.0);
+        std::fill(outb, outb + size_t(Q) * N, 0.0);
+        // stale slots keep their adjoint (the population is carried unchanged)
+        for (int z = 1; z < NZ - 1; ++z)
+            for (int x = 1; x < NX - 1; ++x)
+                for (int i = 0; i < Q; ++i) {
+                    const int sx = x - kCX[i], sz = z - kCZ[i];
+                    if (kCY[i] == 1 && !(inner(sx, sz) && is_jet(sx, sz))) {
+                        const size_t t = id(x, 1, z);
+                        outr[size_t(i) * N + t] = lr[size_t(i) * N + t];
+                        outb[size_t(i) * N + t] = lb[size_t(i) * N + t];
+                    }
+                    if (kCY[i] == -1 && !inner(sx, sz)) {
+                        const size_t t = id(x, NY - 2, z);
+                        outr[size_t(i) * N + t] = lr[size_t(i) * N + t];
+                        outb[size_t(i) * N + t] = lb[size_t(i) * N + t];
+                    }
+                }
+        for (int z = 1; z < NZ - 1; ++z)
+            for (int y = 1; y < NY - 1; ++y)
+                for (int x = 1; x < NX - 1; ++x) {
+                    const size_t g = id(x, y, z);
+                    double m[11], lam[11];
+                    mom(st, g, m);
+                    for (int k = 0; k < 11; ++k) lam[k] = LM[size_t(k) * N + g];
+                    const double lp = Lpsi(x, y, z), rho = m[0] + m[1];
+                    lam[0] += lp * 2.0 * m[1] / (rho * rho);
+                    lam[1] += -lp * 2.0 * m[0] / (rho * rho);
+                    double r[Q], b[Q];
+                    adj_moments<double>(m, lam, r, b);
+                    for (int i = 0; i < Q; ++i) {
+                        outr[size_t(i) * N + g] += r[i];
+                        outb[size_t(i) * N + g] += b[i];
+                    }
+                }
+    }
+};
+
 }  // namespace mcg

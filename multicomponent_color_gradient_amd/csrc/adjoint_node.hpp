@@ -77,6 +77,74 @@ MCG_HD T h3T(int k, int i) {
     }
 }
 
+// Adjoint of the third-order products t_k(Pi, u) of fneqr (lbm.cuh:58-72):
+// adds into l_P and l_u.
+template <class T>
+MCG_HD void t_vjp(const T P[6], T ux, T uy, T uz, const T l_t[7], T l_P[6], T l_u[3]) {
+    l_P[PXX] += l_t[TXXY] * uy + l_t[TXXZ] * uz;
+    l_P[PYY] += l_t[TXYY] * ux + l_t[TYYZ] * uz;
+    l_P[PZZ] += l_t[TXZZ] * ux + l_t[TYZZ] * uy;
+    l_P[PXY] += T(2) * (l_t[TXXY] * ux + l_t[TXYY] * uy + l_t[TXYZ] * uz);
+    l_P[PXZ] += T(2) * (l_t[TXXZ] * ux + l_t[TXZZ] * uz + l_t[TXYZ] * uy);
+    l_P[PYZ] += T(2) * (l_t[TYYZ] * uy + l_t[TYZZ] * uz + l_t[TXYZ] * ux);
+    l_u[0] += l_t[TXXY] * T(2) * P[PXY] + l_t[TXXZ] * T(2) * P[PXZ] + l_t[TXYY] * P[PYY] + l_t[TXZZ] * P[PZZ] +
+              l_t[TXYZ] * T(2) * P[PYZ];
+    l_u[1] += l_t[TXXY] * P[PXX] + l_t[TXYY] * T(2) * P[PXY] + l_t[TYYZ] * T(2) * P[PYZ] + l_t[TYZZ] * P[PZZ] +
+              l_t[TXYZ] * T(2) * P[PXZ];
+    l_u[2] += l_t[TXXZ] * P[PXX] + l_t[TXZZ] * T(2) * P[PXZ] + l_t[TYYZ] * P[PYY] + l_t[TYZZ] * T(2) * P[PYZ] +
+              l_t[TXYZ] * T(2) * P[PXY];
+}
+
+// Equilibrium of direction i in the reference form (lbm.cuh:15-35) and its
+// derivatives: dfeq/drho and dfeq/du_a.
+template <class T>
+MCG_HD T feqT(int i, T rho, T ux, T uy, T uz) {
+    const T cu = T(kCX[i]) * ux + T(kCY[i]) * uy + T(kCZ[i]) * uz;
+    const T usq = ux * ux + uy * uy + uz * uz;
+    return wT<T>(i) * rho * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq + cu * (T(4.5) * cu * cu - T(4.5) * usq));
+}
+template <class T>
+MCG_HD void feq_vjp(int i, T rho, T ux, T uy, T uz, T lam, T &l_rho, T l_u[3]) {
+    const T cu = T(kCX[i]) * ux + T(kCY[i]) * uy + T(kCZ[i]) * uz;
+    const T usq = ux * ux + uy * uy + uz * uz;
+    const T w = wT<T>(i);
+    l_rho += lam * w * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq + cu * (T(4.5) * cu * cu - T(4.5) * usq));
+    const T ac = w * rho * (T(3) + T(9) * cu + T(13.5) * cu * cu - T(4.5) * usq);  // coefficient of c_a
+    const T au = w * rho * (-T(3) - T(9) * cu);                                  // coefficient of u_a
+    l_u[0] += lam * (ac * T(kCX[i]) + au * ux);
+    l_u[1] += lam * (ac * T(kCY[i]) + au * uy);
+    l_u[2] += lam * (ac * T(kCZ[i]) + au * uz);
+}
+
+// Regularised non-equilibrium population (lbm.cuh:37-78) and its adjoint.
+template <class T>
+MCG_HD void fneqr_products(const T P[6], T ux, T uy, T uz, T t[7]) {
+    t[TXXY] = P[PXX] * uy + T(2) * P[PXY] * ux;
+    t[TXXZ] = P[PXX] * uz + T(2) * P[PXZ] * ux;
+    t[TXYY] = P[PYY] * ux + T(2) * P[PXY] * uy;
+    t[TXZZ] = P[PZZ] * ux + T(2) * P[PXZ] * uz;
+    t[TYYZ] = P[PYY] * uz + T(2) * P[PYZ] * uy;
+    t[TYZZ] = P[PZZ] * uy + T(2) * P[PYZ] * uz;
+    t[TXYZ] = T(2) * (P[PXY] * uz + P[PXZ] * uy + P[PYZ] * ux);
+}
+template <class T>
+MCG_HD T fneqrT(int i, const T P[6], T ux, T uy, T uz) {
+    T a2 = T(0), a3 = T(0), t[7];
+    for (int k = 0; k < 6; ++k) a2 += ((k == PXY || k == PYZ || k == PXZ) ? T(2) : T(1)) * h2T<T>(k, i) * P[k];
+    fneqr_products<T>(P, ux, uy, uz, t);
+    for (int k = 0; k < 7; ++k) a3 += h3T<T>(k, i) * t[k];
+    return wT<T>(i) * (T(4.5) * a2 + T(13.5) * a3);
+}
+template <class T>
+MCG_HD void fneqr_vjp(int i, const T P[6], T ux, T uy, T uz, T lam, T l_P[6], T l_u[3]) {
+    const T w = wT<T>(i);
+    for (int k = 0; k < 6; ++k)
+        l_P[k] += lam * w * T(4.5) * ((k == PXY || k == PYZ || k == PXZ) ? T(2) : T(1)) * h2T<T>(k, i);
+    T l_t[7];
+    for (int k = 0; k < 7; ++k) l_t[k] = lam * w * T(13.5) * h3T<T>(k, i);
+    t_vjp<T>(P, ux, uy, uz, l_t, l_P, l_u);
+}
+
 template <class T>
 struct AdjNode {
     T rr, rb, rho, aR, aB, ux, uy, uz, usq;
@@ -292,19 +360,10 @@ MCG_HD AdjOut<T> adj_vjp_fn(const AdjNode<T> &n, MuFn &&mu) {
     l_P[PYY] += -l_tr / T(3);
     l_P[PZZ] += -l_tr / T(3);
     // third-order products
+    t_vjp<T>(n.P, n.ux, n.uy, n.uz, l_t, l_P, l_u);
+    // (the products' adjoint is added into l_P and l_u)
     const T *P = n.P;
-    l_P[PXX] += l_t[TXXY] * n.uy + l_t[TXXZ] * n.uz;
-    l_P[PYY] += l_t[TXYY] * n.ux + l_t[TYYZ] * n.uz;
-    l_P[PZZ] += l_t[TXZZ] * n.ux + l_t[TYZZ] * n.uy;
-    l_P[PXY] += T(2) * (l_t[TXXY] * n.ux + l_t[TXYY] * n.uy + l_t[TXYZ] * n.uz);
-    l_P[PXZ] += T(2) * (l_t[TXXZ] * n.ux + l_t[TXZZ] * n.uz + l_t[TXYZ] * n.uy);
-    l_P[PYZ] += T(2) * (l_t[TYYZ] * n.uy + l_t[TYZZ] * n.uz + l_t[TXYZ] * n.ux);
-    l_u[0] += l_t[TXXY] * T(2) * P[PXY] + l_t[TXXZ] * T(2) * P[PXZ] + l_t[TXYY] * P[PYY] + l_t[TXZZ] * P[PZZ] +
-              l_t[TXYZ] * T(2) * P[PYZ];
-    l_u[1] += l_t[TXXY] * P[PXX] + l_t[TXYY] * T(2) * P[PXY] + l_t[TYYZ] * T(2) * P[PYZ] + l_t[TYZZ] * P[PZZ] +
-              l_t[TXYZ] * T(2) * P[PXZ];
-    l_u[2] += l_t[TXXZ] * P[PXX] + l_t[TXZZ] * T(2) * P[PXZ] + l_t[TYYZ] * P[PYY] + l_t[TYZZ] * T(2) * P[PYZ] +
-              l_t[TXYZ] * T(2) * P[PXY];
+    (void)P;
     // usq
     l_u[0] += T(2) * n.ux * l_usq;
     l_u[1] += T(2) * n.uy * l_usq;

@@ -1007,4 +1007,56 @@ int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const doub
     return MCG_OK;
 }
 
+int mcg_cpu_adjoint_jet(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *fields0,
+                        const double *lam_fr, const double *lam_fb, const double *w_uy, int uy_first, int uy_last,
+                        double *out_fr, double *out_fb, double *out_lam_fr, double *out_lam_fb, double *dparams,
+                        double *dinlet) {
+    if (!c || !fr0 || !fb0 || !fields0) return fail(MCG_E_INVALID, "null pointer");
+    if (int rc = mcg_case_validate(c)) return rc;
+    if (c->y_mode != MCG_Y_JET || c->wrap_gradient) return fail(MCG_E_UNSUPPORTED, "the jet adjoint covers y_mode jet with the reference gradient");
+    if (c->viscosity_mode != MCG_VISC_SINGLE || c->sponge_K != 0.0)
+        return fail(MCG_E_UNSUPPORTED, "the CPU adjoint assumes a uniform relaxation rate");
+    if (nsteps < 0) return fail(MCG_E_INVALID, "nsteps < 0");
+    CpuJetAdjoint J;
+    J.c = *c;
+    J.NX = c->NX;
+    J.NY = c->NY;
+    J.NZ = c->NZ;
+    J.N = size_t(c->NX) * c->NY * c->NZ;
+    J.omega = 1.0 / c->taub;
+    J.A = c->A_override > 0.0 ? c->A_override : c->sigma / (4.0 / 9.0) / c->taub;
+    J.beta = c->beta_recolor;
+    const size_t N = J.N, n = size_t(Q) * N, nxz = size_t(c->NX) * c->NZ;
+    J.gpsi.resize(N);
+    for (size_t g = 0; g < N; ++g) J.gpsi[g] = (fields0[g] - fields0[N + g]) / (fields0[g] + fields0[N + g]);
+    std::vector<std::vector<double>> R(size_t(nsteps) + 1, std::vector<double>(n)), B(size_t(nsteps) + 1, std::vector<double>(n)),
+        G(size_t(nsteps) + 1, std::vector<double>(5 * nxz, 0.0));
+    std::copy(fr0, fr0 + n, R[0].begin());
+    std::copy(fb0, fb0 + n, B[0].begin());
+    for (int z = 1; z < c->NZ - 1; ++z)
+        for (int x = 1; x < c->NX - 1; ++x)
+            for (int k = 0; k < 5; ++k)
+                G[0][k * nxz + size_t(z) * c->NX + x] = fields0[k * N + J.id(x, c->NY - 1, z)];
+    for (int k = 0; k < nsteps; ++k) J.step(k, R[k].data(), B[k].data(), G[k].data(), R[k + 1].data(), B[k + 1].data(), G[k + 1].data());
+    if (out_fr) std::copy(R[nsteps].begin(), R[nsteps].end(), out_fr);
+    if (out_fb) std::copy(B[nsteps].begin(), B[nsteps].end(), out_fb);
+    if (!lam_fr || !lam_fb) return MCG_OK;
+    std::vector<double> lr(lam_fr, lam_fr + n), lb(lam_fb, lam_fb + n), tr(n), tb(n), lg(5 * nxz, 0.0), tg(5 * nxz);
+    std::vector<double> din(3 * nxz, 0.0);
+    double dp[4] = {0, 0, 0, 0};
+    for (int k = nsteps - 1; k >= 0; --k) {
+        const double *w = (w_uy && k >= uy_first && k <= uy_last) ? w_uy : nullptr;
+        J.adjoint(k, R[k].data(), B[k].data(), G[k].data(), lr.data(), lb.data(), lg.data(), tr.data(), tb.data(), tg.data(), dp,
+                  din.data(), w);
+        lr.swap(tr);
+        lb.swap(tb);
+        lg.swap(tg);
+    }
+    if (out_lam_fr) std::copy(lr.begin(), lr.end(), out_lam_fr);
+    if (out_lam_fb) std::copy(lb.begin(), lb.end(), out_lam_fb);
+    if (dparams) std::copy(dp, dp + 4, dparams);
+    if (dinlet) std::copy(din.begin(), din.end(), dinlet);
+    return MCG_OK;
+}
+
 }  // extern "C"
