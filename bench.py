@@ -111,6 +111,32 @@ def adjoint_ratio(steps: int, every: int) -> dict:
             "ratio": round(bwd_ms / fwd_ms, 3)}
 
 
+def adjoint_jet_ratio(steps: int, every: int) -> dict:
+    """The same ratio on a 128x256x128 perturbed jet (R=8, tanh inlet, 1%
+    transverse perturbation): inlet and convective outlet differentiated, the
+    outlet state carried, a u_y seed on the centre line over every step."""
+    import numpy as np
+
+    import multicomponent_color_gradient_amd as m
+
+    c = m.perturbed_jet_case(128, 256, 128, jet_radius=8.0)
+    s = m.Solver(c)
+    s.init_jet()
+    s.step(10)
+    fwd_ms = s.time_steps(steps)
+    w = np.zeros(s.shape, np.float32)
+    w[int(c.jet_z0), 1:-1, int(c.jet_x0)] = 1.0
+    s.adjoint_set_uy_seed(w, 0, steps - 1)
+    s.adjoint_record(steps, every)
+    lam = np.zeros((27,) + s.shape, np.float32)
+    s.adjoint_backward(lam, lam)
+    bwd_ms = s.adjoint_last_ms()
+    s.close()
+    return {"workload": "perturbed jet 128x256x128, R=8, U 0.05, tanh inlet, perturbation 0.01", "steps": steps,
+            "checkpoint_every": every, "forward_ms": round(fwd_ms, 3), "backward_ms": round(bwd_ms, 3),
+            "ratio": round(bwd_ms / fwd_ms, 3)}
+
+
 def cpu_baseline(case, init: str, budget_s: float):
     """FP64 OpenMP oracle on the same grid: 1 warm-up step, then steps until budget_s."""
     from oracle import oracle as orc
@@ -230,6 +256,7 @@ def main():
                                            "bytes_per_lu": r16["bpl"]}}
     if not args.no_adjoint and args.storage == "fp32":
         out["adjoint"] = adjoint_ratio(args.adjoint_steps, args.adjoint_every)
+        out["adjoint_jet"] = adjoint_jet_ratio(args.adjoint_steps, args.adjoint_every)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, init, args.cpu_seconds)
     else:

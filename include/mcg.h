@@ -212,17 +212,37 @@ int mcg_checkpoint_save(mcg_solver *s, void *host_buf, uint64_t nbytes);
 int mcg_checkpoint_load(mcg_solver *s, const void *host_buf, uint64_t nbytes);
 
 /* ---- adjoint (extension: the reference has no adjoint) ---------------- */
-/* Discrete adjoint of the step for the periodic domain (y_mode periodic,
- * wrap_gradient 1), single relaxation rate, FP32 populations.  The state is
- * the populations in the reference layout (what mcg_copy_populations returns:
- * f_i(x) after streaming); the parameters are A (the perturbation amplitude,
- * A_override or sigma/(4 cs^4 taub)) and beta.  Bulk/interface switches and
- * the |F| <= 1e-4 cut are held at the forward state. */
+/* Discrete adjoint of the step, single relaxation rate, FP32 populations, for
+ * the periodic domain (y_mode periodic, wrap_gradient 1) and for jet cases
+ * (y_mode jet, wrap_gradient 0, sponge_K 0: the inlet and the convective
+ * outlet are differentiated and the outlet state is carried between steps).
+ * The state is the populations in the reference layout (what
+ * mcg_copy_populations returns: f_i(x) after streaming); the parameters are A
+ * (the perturbation amplitude, A_override or sigma/(4 cs^4 taub)) and beta,
+ * and for jet cases jet_velocity, perturbation_amp and the inlet velocity of
+ * every jet column.  Bulk/interface switches, the |F| <= 1e-4 cut and the
+ * outlet's clamp are held at the forward state. */
 
 /* Run nsteps forward steps from the current state and keep a copy of the
- * populations every checkpoint_every steps (device memory: ceil(nsteps/K) +
- * K + 3 population sets during mcg_adjoint_backward). */
+ * state (populations; jet cases also the inlet/outlet arrays) every
+ * checkpoint_every steps (device memory: ceil(nsteps/K) + K + 3 population
+ * sets during mcg_adjoint_backward).  Jet cases: right after an
+ * initialisation the first step must follow mcg_init_jet. */
 int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every);
+
+/* Jet cases: add sum over the moments passes of recorded steps first..last
+ * (0 = the first recorded step) of sum_x w_uy(x) u_y(x) to the objective of
+ * mcg_adjoint_backward (w_uy: NX*NY*NZ floats, x fastest; NULL clears it).
+ * Kept until cleared or mcg_adjoint_release.  No counterpart in the
+ * reference (an extension). */
+int mcg_adjoint_set_uy_seed(mcg_solver *s, const float *w_uy, int first, int last);
+
+/* Jet cases, after mcg_adjoint_backward: dparams = {dJ/djet_velocity,
+ * dJ/dperturbation_amp} (FP64 sums) and dinlet = dJ/de for the inlet velocity
+ * e of each jet column summed over the recorded steps (3 x NX*NZ doubles,
+ * component-major, zero outside the jet).  Either may be NULL.  No counterpart
+ * in the reference (an extension). */
+int mcg_adjoint_inlet_gradient(const mcg_solver *s, double *dparams, double *dinlet);
 
 /* Given lam = dJ/df at the final state (reference layout, 27 x NX*NY*NZ per
  * colour; ghost slots ignored), return dJ/df at the recorded initial state

@@ -138,6 +138,8 @@ SIGNATURES = [
     ("mcg_adjoint_backward", c_int, [_PSOLVER, _F, _F, _F, _F, _D]),
     ("mcg_adjoint_last_ms", c_int, [_PSOLVER, _D]),
     ("mcg_adjoint_release", c_int, [_PSOLVER]),
+    ("mcg_adjoint_set_uy_seed", c_int, [_PSOLVER, _F, c_int, c_int]),
+    ("mcg_adjoint_inlet_gradient", c_int, [_PSOLVER, _D, _D]),
     ("mcg_cpu_adjoint", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     ("mcg_cpu_adjoint_jet", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, c_int, c_int, _D, _D, _D, _D, _D, _D]),
     ("mcg_time_steps", c_int, [_PSOLVER, c_int, _D]),

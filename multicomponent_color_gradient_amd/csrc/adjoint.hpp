@@ -51,13 +51,50 @@ __global__ __launch_bounds__(256) void k_psi_plane(DevParams P, const float *__r
     ps[o] = mom_psi(P, mom, o);
 }
 
+// Jet-case inputs of the collision adjoint.
+struct AdjJet {
+    const float *psf;   // psi over the full grid (k_psi_full)
+    const float *w_uy;  // seed on u_y (full grid) or null
+};
+
+// Jet cases: psi at every position of the full grid as the colour gradient
+// reads it (forces.cuh:31-33, no wrap): interior nodes from the moments, the
+// inner columns of the outlet plane NY-1 from this step's outlet state gBn,
+// every other ghost position from the ghost densities.
+__global__ __launch_bounds__(256) void k_psi_full(DevParams P, const float *__restrict__ mom, const float *__restrict__ gBn,
+                                                  const float *__restrict__ grr, const float *__restrict__ grb,
+                                                  float *__restrict__ psf) {
+    const int x = int(blockIdx.x) * 64 + int(threadIdx.x);
+    const int y = int(blockIdx.y) * 4 + int(threadIdx.y);
+    const int z = int(blockIdx.z);
+    if (x >= P.NX || y >= P.NY) return;
+    const size_t g = gidx(P, x, y, z);
+    const bool inner = x >= 1 && x <= P.NX - 2 && z >= 1 && z <= P.NZ - 2;
+    float v;
+    if (inner && y >= 1 && y <= P.NY - 2) {
+        v = mom_psi(P, mom, midx(P, x, y, z));
+    } else if (inner && y == P.NY - 1) {
+        const size_t nxz = size_t(P.NX) * P.NZ, o = size_t(z) * P.NX + x;
+        v = psi(gBn[o], gBn[nxz + o]);
+    } else {
+        v = psi(grr[g], grb[g]);
+    }
+    psf[g] = v;
+}
+
 // Adjoint of the collision at every interior node: lam_m (11 planes),
 // lam_F (3 planes), and the workgroup's sums of dJ/dA, dJ/dbeta in
 // part[2 b], part[2 b + 1] (b = linear workgroup id; reduced by k_adj_sum in
 // a fixed order, so the FP64 parameter gradients are deterministic).
+// Jet cases (JET): the colour gradient reads psi without wrap, ghost
+// positions from the ghost densities except the outlet plane NY-1 (its inner
+// columns hold the outlet state gBn of this step); populations that stream
+// through the planes y=0 / y=NY-1 leave the domain; w_uy (optional) seeds u_y.
+template <bool JET>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MCG_ADJ_WAVES_PER_EU)))
 void k_adj_node(DevParams P, const float *__restrict__ mom, const float *__restrict__ ps,
-                const float *__restrict__ lam_in, float *__restrict__ lam_m, float *__restrict__ lam_F, double *part) {
+                const float *__restrict__ lam_in, float *__restrict__ lam_m, float *__restrict__ lam_F, double *part,
+                AdjJet J) {
     __shared__ double wsum[2][4];
     const int x = 1 + int(blockIdx.x) * 64 + int(threadIdx.x);
     const int y = 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
@@ -71,6 +108,7 @@ void k_adj_node(DevParams P, const float *__restrict__ mom, const float *__restr
         float Fx = 0.f, Fy = 0.f, Fz = 0.f;
         auto phi = [&](auto Ic) -> float {
             constexpr int I = decltype(Ic)::value;
+            if constexpr (JET) return J.psf[gidx(P, x + kCX[I], y + kCY[I], z + kCZ[I])];
             return ps[midx(P, wrapN(x + kCX[I], P.NX), wrapN(y + kCY[I], P.NY), wrapN(z + kCZ[I], P.NZ))];
         };
         color_gradient(phi, Fx, Fy, Fz);
@@ -78,11 +116,17 @@ void k_adj_node(DevParams P, const float *__restrict__ mom, const float *__restr
         const AdjNode<float> n = adj_setup<float>(m, Fx, Fy, Fz, A_node(P, m[0], m[1]), P.beta, omega);
         // mu_i(x) = lambda_i(x + c_i), loaded where the pair needs it
         const AdjOut<float> a = adj_vjp_fn<float>(n, [&](int i, float &r, float &b) {
-            const uint32_t q = pidx(P, wrapN(x + kCX[i], P.NX), wrapN(y + kCY[i], P.NY), wrapN(z + kCZ[i], P.NZ));
+            const int ty = JET ? y + kCY[i] : wrapN(y + kCY[i], P.NY);
+            if (JET && (ty < 1 || ty > P.NY - 2)) {  // leaves through the inlet or outlet plane
+                r = b = 0.0f;
+                return;
+            }
+            const uint32_t q = pidx(P, wrapN(x + kCX[i], P.NX), ty, wrapN(z + kCZ[i], P.NZ));
             r = lam_in[uint32_t(i) * uint32_t(P.PX) + q];
             b = lam_in[uint32_t(Q + i) * uint32_t(P.PX) + q];
         });
         for (int k = 0; k < 11; ++k) lam_m[size_t(k) * P.plane + o] = a.m[k];
+        if (JET && J.w_uy) lam_m[3 * size_t(P.plane) + o] += J.w_uy[gidx(P, x, y, z)];
         for (int k = 0; k < 3; ++k) lam_F[size_t(k) * P.plane + o] = a.F[k];
         gA = double(a.A);
         gB = double(a.beta);
@@ -101,18 +145,27 @@ void k_adj_node(DevParams P, const float *__restrict__ mom, const float *__restr
     }
 }
 
-// acc[0..1] += the sums of the nb workgroup partials (one workgroup, fixed order)
-__global__ __launch_bounds__(256) void k_adj_sum(const double *__restrict__ part, size_t nb, double *acc) {
-    __shared__ double sh[2][256];
-    double a = 0.0, b = 0.0;
-    for (size_t k = threadIdx.x; k < nb; k += 256) {
-        a += part[2 * k];
-        b += part[2 * k + 1];
+// acc[0..1] += the sums of the nb workgroup partials (one workgroup of 1024
+// threads, four independent accumulators each, fixed order: deterministic)
+constexpr int kSumThreads = 1024;
+__global__ __launch_bounds__(kSumThreads) void k_adj_sum(const double *__restrict__ part, size_t nb, double *acc) {
+    __shared__ double sh[2][kSumThreads];
+    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+    size_t k = threadIdx.x;
+    for (; k + 3 * kSumThreads < nb; k += 4 * kSumThreads) {
+        for (int u = 0; u < 4; ++u) {
+            a[u] += part[2 * (k + u * kSumThreads)];
+            b[u] += part[2 * (k + u * kSumThreads) + 1];
+        }
     }
-    sh[0][threadIdx.x] = a;
-    sh[1][threadIdx.x] = b;
+    for (; k < nb; k += kSumThreads) {
+        a[0] += part[2 * k];
+        b[0] += part[2 * k + 1];
+    }
+    sh[0][threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
+    sh[1][threadIdx.x] = (b[0] + b[1]) + (b[2] + b[3]);
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (int h = kSumThreads / 2; h > 0; h >>= 1) {
         if (int(threadIdx.x) < h) {
             sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
             sh[1][threadIdx.x] += sh[1][threadIdx.x + h];
@@ -126,16 +179,27 @@ __global__ __launch_bounds__(256) void k_adj_sum(const double *__restrict__ part
 }
 
 // Colour-gradient transpose, psi derivative and moments-pass adjoint:
-// lambda^n at every interior node.
+// lambda^n at every interior node.  Jet cases: the transpose runs over the
+// interior neighbours without wrap, and the stale inlet/outlet slots (never
+// rewritten, so carried unchanged) pass their adjoint through.
+template <bool JET>
 __global__ __launch_bounds__(256) void k_adj_pop(DevParams P, const float *__restrict__ mom,
                                                  const float *__restrict__ lam_m, const float *__restrict__ lam_F,
-                                                 float *__restrict__ lam_out) {
+                                                 const float *__restrict__ lam_in, float *__restrict__ lam_out) {
     MCG_INTERIOR_XYZ(P, x, y, z)
     const uint32_t o = midx(P, x, y, z);
     float Lpsi = 0.f;
     unroll<1, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
-        const uint32_t q = midx(P, wrapN(x - kCX[I], P.NX), wrapN(y - kCY[I], P.NY), wrapN(z - kCZ[I], P.NZ));
+        int sx = x - kCX[I], sy = y - kCY[I], sz = z - kCZ[I];
+        if constexpr (JET) {
+            if (sx < 1 || sx > P.NX - 2 || sy < 1 || sy > P.NY - 2 || sz < 1 || sz > P.NZ - 2) return;
+        } else {
+            sx = wrapN(sx, P.NX);
+            sy = wrapN(sy, P.NY);
+            sz = wrapN(sz, P.NZ);
+        }
+        const uint32_t q = midx(P, sx, sy, sz);
         float d = 0.f;
         if constexpr (kCX[I] != 0) d += float(kCX[I]) * lam_F[q];
         if constexpr (kCY[I] != 0) d += float(kCY[I]) * lam_F[size_t(P.plane) + q];
@@ -153,10 +217,175 @@ __global__ __launch_bounds__(256) void k_adj_pop(DevParams P, const float *__res
     lam[1] += -Lpsi * 2.0f * m[0] * inv2;
     float lr[Q], lb[Q];
     adj_moments<float>(m, lam, lr, lb);
+    if constexpr (JET) {
+        auto inner = [&](int a, int c) { return a >= 1 && a <= P.NX - 2 && c >= 1 && c <= P.NZ - 2; };
+        for (int i = 0; i < Q; ++i) {
+            const int gx = x - kCX[i], gz = z - kCZ[i];
+            const bool stale = (y == 1 && kCY[i] == 1 && !(inner(gx, gz) && is_jet(P, gx, gz))) ||
+                               (y == P.NY - 2 && kCY[i] == -1 && !inner(gx, gz));
+            if (stale) {
+                lr[i] += lam_in[nrow(P, 0, i, x, y, z)];
+                lb[i] += lam_in[nrow(P, 1, i, x, y, z)];
+            }
+        }
+    }
     for (int i = 0; i < Q; ++i) {
         __builtin_nontemporal_store(lr[i], &lam_out[nrow(P, 0, i, x, y, z)]);
         __builtin_nontemporal_store(lb[i], &lam_out[nrow(P, 1, i, x, y, z)]);
     }
+}
+
+// Inlet velocity of jet column (x,z) at step n with its derivatives with
+// respect to jet_velocity (dU) and perturbation_amp (damp); see inlet_velocity.
+__device__ __forceinline__ void inlet_velocity_grad(const DevParams &P, uint64_t step, int x, int z, float e[3], float dU[3],
+                                                    float damp[3]) {
+    float prof = 1.0f;
+    e[0] = e[2] = dU[0] = dU[2] = damp[0] = damp[1] = damp[2] = 0.0f;
+    if (P.inlet_profile == MCG_INLET_TANH) {
+        const float dx = float(x) - P.jet_x0, dz = float(z) - P.jet_z0;
+        prof = 0.5f * (1.0f - tanhf((sqrtf(dx * dx + dz * dz) - P.jet_R) / P.inlet_w));
+        const float sx = 2.0f * inlet_xi(P.pert_seed, step, x, z, P.NX, 1) - 1.0f;
+        const float sz = 2.0f * inlet_xi(P.pert_seed, step, x, z, P.NX, 2) - 1.0f;
+        const float a = P.pert_amp * P.jet_U;
+        e[0] = a * sx;
+        e[2] = a * sz;
+        dU[0] = P.pert_amp * sx;
+        dU[2] = P.pert_amp * sz;
+        damp[0] = P.jet_U * sx;
+        damp[2] = P.jet_U * sz;
+    }
+    e[1] = P.jet_U * prof;
+    dU[1] = prof;
+}
+
+// block sum of two doubles into part[2 b], part[2 b + 1] (256 threads)
+__device__ __forceinline__ void block_part2(double a, double b, double *part) {
+    __shared__ double sh[2][4];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int wv = int(threadIdx.x) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][wv] = a;
+        sh[1][wv] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+        part[2 * blockIdx.x + 1] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+    }
+}
+
+// Boundary adjoint of a jet step, one thread per inner column (x,z); runs
+// after k_adj_node (reads lam_F, adds to lam_m at (x,1,z) and (x,NY-2,z),
+// which only this thread touches).
+//  * convective outlet (jet/boundary_conditions.cuh:163-233): the 9 emitted
+//    populations and the update of the outlet state; lgB_out = adjoint of
+//    this step's outlet state gBn, from lgB_in = adjoint of the next one;
+//  * psi of the outlet ghost plane, read by the colour gradient of row NY-2:
+//    its transpose, into lgB_out;
+//  * inlet of a jet column (jet/boundary_conditions.cuh:97-148): adds to
+//    dJ/de of the column (FP64, summed over steps) and the dU, damp partials
+//    part[2 b], part[2 b + 1].
+__global__ __launch_bounds__(256) void k_adj_cols(DevParams P, const float *__restrict__ mom,
+                                                  const float *__restrict__ lam_in, const float *__restrict__ lam_F,
+                                                  float *__restrict__ lam_m, const float *__restrict__ gBn,
+                                                  const float *__restrict__ lgB_in, float *__restrict__ lgB_out,
+                                                  double *__restrict__ dinlet, double *part, unsigned long long step) {
+    const int c = int(blockIdx.x) * 256 + int(threadIdx.x);
+    const int nx = P.NX - 2;
+    const int x = 1 + c % nx, z = 1 + c / nx;
+    double pU = 0.0, pA = 0.0;
+    if (z <= P.NZ - 2) {
+        const size_t nxz = size_t(P.NX) * P.NZ, oc = size_t(z) * P.NX + x;
+        // outlet
+        {
+            const uint32_t o = midx(P, x, P.NY - 2, z);
+            float m[11];
+            for (int k = 0; k < 11; ++k) m[k] = mom[size_t(k) * P.plane + o];
+            const float uc = clamp01(m[3]);
+            float gB[5], g1[5];
+            for (int k = 0; k < 5; ++k) {
+                gB[k] = gBn[k * nxz + oc];
+                g1[k] = gB[k] - uc * (gB[k] - m[k]);
+            }
+            const float rhoT = g1[0] + g1[1];
+            const float inv = 1.0f / rhoT;
+            const float aR = g1[0] * inv, aB = g1[1] * inv;
+            const float oms = 1.0f - omega_node(P, P.NY - 2, g1[0], g1[1]);
+            float l_aR = 0.f, l_aB = 0.f, l_rhoT = 0.f, l_uB[3] = {0.f, 0.f, 0.f}, lP[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f},
+                  lu[3] = {0.f, 0.f, 0.f};
+            for (int k = 0; k < 9; ++k) {
+                const int i = kDown[k], tx = x + kCX[i], tz = z + kCZ[i];
+                if (tx < 1 || tx > P.NX - 2 || tz < 1 || tz > P.NZ - 2) continue;
+                const float lrk = lam_in[nrow(P, 0, i, tx, P.NY - 2, tz)], lbk = lam_in[nrow(P, 1, i, tx, P.NY - 2, tz)];
+                const float g = feqT<float>(i, rhoT, g1[2], g1[3], g1[4]) + oms * fneqrT<float>(i, m + 5, m[2], m[3], m[4]);
+                l_aR += g * lrk;
+                l_aB += g * lbk;
+                const float lg = aR * lrk + aB * lbk;
+                feq_vjp<float>(i, rhoT, g1[2], g1[3], g1[4], lg, l_rhoT, l_uB);
+                fneqr_vjp<float>(i, m + 5, m[2], m[3], m[4], oms * lg, lP, lu);
+            }
+            const float q = (g1[0] * l_aR + g1[1] * l_aB) * inv * inv;
+            float L[5];
+            L[0] = l_aR * inv + l_rhoT - q + lgB_in[oc];
+            L[1] = l_aB * inv + l_rhoT - q + lgB_in[nxz + oc];
+            for (int k = 0; k < 3; ++k) L[2 + k] = l_uB[k] + lgB_in[(2 + k) * nxz + oc];
+            float l_uc = 0.f, lo[5];
+            for (int k = 0; k < 5; ++k) {
+                lo[k] = (1.0f - uc) * L[k];
+                lam_m[size_t(k) * P.plane + o] += uc * L[k];
+                l_uc += -(gB[k] - m[k]) * L[k];
+            }
+            if (m[3] > 0.0f && m[3] < 1.0f) lam_m[3 * size_t(P.plane) + o] += l_uc;
+            for (int k = 0; k < 6; ++k) lam_m[size_t(5 + k) * P.plane + o] += lP[k];
+            for (int k = 0; k < 3; ++k) lam_m[size_t(2 + k) * P.plane + o] += lu[k];
+            // outlet ghost psi (c_y = +1 neighbours of row NY-2 read it)
+            float Lpsi = 0.f;
+            unroll<1, Q>([&](auto Ic) {
+                constexpr int I = decltype(Ic)::value;
+                if constexpr (kCY[I] == 1) {
+                    const int sx = x - kCX[I], sz = z - kCZ[I];
+                    if (sx < 1 || sx > P.NX - 2 || sz < 1 || sz > P.NZ - 2) return;
+                    const uint32_t qn = midx(P, sx, P.NY - 2, sz);
+                    float d = lam_F[size_t(P.plane) + qn];
+                    if constexpr (kCX[I] != 0) d += float(kCX[I]) * lam_F[qn];
+                    if constexpr (kCZ[I] != 0) d += float(kCZ[I]) * lam_F[2 * size_t(P.plane) + qn];
+                    Lpsi += inv_cs2 * w(I) * d;
+                }
+            });
+            const float rho = gB[0] + gB[1];
+            lo[0] += Lpsi * 2.0f * gB[1] / (rho * rho);
+            lo[1] += -Lpsi * 2.0f * gB[0] / (rho * rho);
+            for (int k = 0; k < 5; ++k) lgB_out[k * nxz + oc] = lo[k];
+        }
+        // inlet
+        if (is_jet(P, x, z)) {
+            const uint32_t o = midx(P, x, 1, z);
+            float m[11];
+            for (int k = 0; k < 11; ++k) m[k] = mom[size_t(k) * P.plane + o];
+            // jet node: red share 0, blue share 1, total density rhob0
+            const float rT = P.rhob0;
+            const float oms = 1.0f - P.omega_b;
+            float e[3], dU[3], da[3], le[3] = {0.f, 0.f, 0.f}, lP[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, lu[3] = {0.f, 0.f, 0.f};
+            inlet_velocity_grad(P, step, x, z, e, dU, da);
+            for (int k = 0; k < 9; ++k) {
+                const int i = kUp[k], tx = x + kCX[i], tz = z + kCZ[i];
+                if (tx < 1 || tx > P.NX - 2 || tz < 1 || tz > P.NZ - 2) continue;
+                const float lg = lam_in[nrow(P, 1, i, tx, 1, tz)];
+                float l_rho = 0.f;
+                feq_vjp<float>(i, rT, e[0], e[1], e[2], lg, l_rho, le);
+                fneqr_vjp<float>(i, m + 5, m[2], m[3], m[4], oms * lg, lP, lu);
+            }
+            for (int k = 0; k < 6; ++k) lam_m[size_t(5 + k) * P.plane + o] += lP[k];
+            for (int k = 0; k < 3; ++k) lam_m[size_t(2 + k) * P.plane + o] += lu[k];
+            for (int k = 0; k < 3; ++k) {
+                dinlet[k * nxz + oc] += double(le[k]);
+                pU += double(le[k]) * double(dU[k]);
+                pA += double(le[k]) * double(da[k]);
+            }
+        }
+    }
+    block_part2(pU, pA, part);
 }
 
 // reference layout (i*Ncells + id, ghosts included) <-> node layout rows

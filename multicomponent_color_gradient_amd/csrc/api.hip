@@ -86,33 +86,50 @@ struct mcg_solver {
     bool started = false;
     int64_t start_step = 0, samples = 0;
     InitGeo geo{};
+    int init_kind = -1;
     size_t bytes = 0;
 
     // adjoint: checkpoints of the recorded forward, the states of one segment,
     // the adjoint fields and the FP64 parameter sums
     bool stats_frozen = false;  // recomputation must not touch the statistics
+    float *rec_mom = nullptr;   // if set, one_step stores the moments of its source state here
     bool adj_recorded = false;
     int adj_nsteps = 0, adj_every = 0;
     int64_t adj_step0 = 0;
     std::vector<void *> adj_ckpt, adj_seg;
-    void *adj_keep[2] = {nullptr, nullptr};
+    float *adj_mom = nullptr;  // moments of the states of one segment
     float *adj_lam[2] = {nullptr, nullptr};
     float *adj_lm = nullptr, *adj_lF = nullptr, *adj_ps = nullptr;
     double *adj_acc = nullptr, *adj_part = nullptr;
     double adj_ms = 0.0;  // device time of the last backward sweep (recomputation + adjoint)
+    // jet cases: adjoint of the outlet state, dJ/d(inlet velocity) per column,
+    // the u_y seed and the steps it applies to
+    float *adj_lg[2] = {nullptr, nullptr};
+    double *adj_din = nullptr;
+    float *adj_psf = nullptr;  // psi over the full grid
+    float *adj_wuy = nullptr;
+    int adj_uy_first = 0, adj_uy_last = -1;
+    double adj_inlet[2] = {0.0, 0.0};  // dJ/djet_velocity, dJ/dperturbation_amp of the last sweep
 
     void adj_release() {
         for (void *p : adj_ckpt) (void)hipFree(p);
         for (void *p : adj_seg) (void)hipFree(p);
         adj_ckpt.clear();
         adj_seg.clear();
-        for (void *p : {adj_keep[0], adj_keep[1], (void *)adj_lam[0], (void *)adj_lam[1], (void *)adj_lm, (void *)adj_lF,
-                        (void *)adj_ps, (void *)adj_acc, (void *)adj_part})
+        for (void *p : {(void *)adj_mom, (void *)adj_lam[0], (void *)adj_lam[1], (void *)adj_lm, (void *)adj_lF,
+                        (void *)adj_ps, (void *)adj_acc, (void *)adj_part, (void *)adj_lg[0], (void *)adj_lg[1],
+                        (void *)adj_din, (void *)adj_psf, (void *)adj_wuy})
             if (p) (void)hipFree(p);
-        adj_keep[0] = adj_keep[1] = nullptr;
+        adj_mom = nullptr;
         adj_lam[0] = adj_lam[1] = nullptr;
         adj_lm = adj_lF = adj_ps = nullptr;
         adj_acc = adj_part = nullptr;
+        adj_lg[0] = adj_lg[1] = nullptr;
+        adj_din = nullptr;
+        adj_psf = nullptr;
+        adj_wuy = nullptr;
+        adj_uy_first = 0;
+        adj_uy_last = -1;
         adj_recorded = false;
     }
 
@@ -121,6 +138,15 @@ struct mcg_solver {
     size_t pop_bytes() const { return size_t(P.rowstride) * size_t(P.IY) * size_t(P.IZ) * sizeof(float); }
     size_t bnd_bytes() const { return size_t(18) * nxz() * sizeof(float); }
     size_t gB_bytes() const { return size_t(5) * nxz() * sizeof(float); }
+    size_t mom_set_bytes() const { return al256(size_t(11) * P.plane * sizeof(float)); }
+    // one recorded forward state: the populations and, for jet cases, the
+    // inlet and outlet source arrays and the outlet state (256-byte aligned parts)
+    static size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+    bool jet() const { return c.y_mode == MCG_Y_JET; }
+    size_t state_bytes() const { return al256(pop_bytes()) + (jet() ? 2 * al256(bnd_bytes()) + al256(gB_bytes()) : 0); }
+    float *st_bin(void *st) const { return (float *)((char *)st + al256(pop_bytes())); }
+    float *st_bout(void *st) const { return (float *)((char *)st + al256(pop_bytes()) + al256(bnd_bytes())); }
+    float *st_gB(void *st) const { return (float *)((char *)st + al256(pop_bytes()) + 2 * al256(bnd_bytes())); }
     size_t bins() const { return size_t(c.NY) * P.NR; }
 
     dim3 grid_interior() const { return dim3(unsigned(P.PX / 64), unsigned((P.IY + 3) / 4), unsigned(P.IZ)); }
@@ -139,6 +165,7 @@ struct mcg_solver {
         B.ghost_rr = grr;
         B.ghost_rb = grb;
         B.mom = mom;
+        B.mom_out = rec_mom;
         B.stats = stats;
         B.count = count;
         B.do_stats = do_stats ? 1 : 0;
@@ -277,6 +304,7 @@ int do_init(mcg_solver *s, int kind) {
     HIP_TRY(hipMemsetAsync(s->count, 0, sizeof(unsigned long long) * s->bins(), s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->step = 0;
+    s->init_kind = kind;
     s->initialized = true;
     s->moments_valid = true;  // all-zero interior moments before the first step (memory.cuh:85-94)
     s->inlet_reset_pending = (c.y_mode == MCG_Y_JET);
@@ -287,6 +315,18 @@ int do_init(mcg_solver *s, int kind) {
 }
 
 int one_step(mcg_solver *s) {
+    // two-pass kernel: the moments pass itself writes the recorded moments
+    struct MomSwap {
+        mcg_solver *s;
+        float *keep;
+        ~MomSwap() {
+            if (keep) s->mom = keep;
+        }
+    } swap_guard{s, nullptr};
+    if (s->rec_mom && s->kernel == MCG_KERNEL_TWOPASS) {
+        swap_guard.keep = s->mom;
+        s->mom = s->rec_mom;
+    }
     // update_radial_statistics_start, meanF/meanFieldsRT.cuh:173-189
     if (!s->stats_frozen && !s->started && s->step >= s->c.NSTATS_START_STEP) {
         s->started = true;
@@ -797,10 +837,26 @@ int mcg_device_bytes(const mcg_solver *s, uint64_t *nbytes) {
 // ---------------------------------------------------------------- adjoint
 
 static int adj_supported(const mcg_case &c, int storage) {
-    if (c.y_mode != MCG_Y_PERIODIC || !c.wrap_gradient)
-        return fail(MCG_E_UNSUPPORTED, "the adjoint covers the periodic domain (y_mode periodic, wrap_gradient 1)");
+    if (c.y_mode == MCG_Y_JET) {
+        if (c.wrap_gradient) return fail(MCG_E_UNSUPPORTED, "the jet adjoint covers the reference gradient (wrap_gradient 0)");
+        if (c.sponge_K != 0.0) return fail(MCG_E_UNSUPPORTED, "the jet adjoint assumes a uniform relaxation rate (sponge_K = 0)");
+    } else if (c.y_mode != MCG_Y_PERIODIC || !c.wrap_gradient) {
+        return fail(MCG_E_UNSUPPORTED, "the adjoint covers jet cases and the periodic domain (y_mode periodic, wrap_gradient 1)");
+    }
     if (c.viscosity_mode != MCG_VISC_SINGLE) return fail(MCG_E_UNSUPPORTED, "the adjoint covers the single relaxation rate");
     if (storage != MCG_STORE_FP32) return fail(MCG_E_UNSUPPORTED, "the adjoint runs on FP32 populations");
+    return MCG_OK;
+}
+
+// copy the solver's current state (populations and, for jet cases, the
+// boundary arrays) into a recorded state buffer
+static int save_state(mcg_solver *s, void *st) {
+    HIP_TRY(hipMemcpyAsync(st, s->f[s->cur], s->pop_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    if (s->jet()) {
+        HIP_TRY(hipMemcpyAsync(s->st_bin(st), s->bin[s->cur], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->st_bout(st), s->bout[s->cur], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->st_gB(st), s->gB[s->cur], s->gB_bytes(), hipMemcpyDeviceToDevice, s->stream));
+    }
     return MCG_OK;
 }
 
@@ -808,11 +864,22 @@ int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every) {
     if (int rc = check_ready(s)) return rc;
     if (int rc = adj_supported(s->c, s->storage)) return rc;
     if (nsteps < 1 || checkpoint_every < 1) return fail(MCG_E_INVALID, "nsteps and checkpoint_every must be >= 1");
+    // the recomputation reads the ghost densities as constants; the first
+    // step's inlet reset keeps them only after the jet initialisation
+    if (s->inlet_reset_pending && s->init_kind != kInitJet)
+        return fail(MCG_E_UNSUPPORTED, "the adjoint of the first jet step needs the jet initialisation (take one step first)");
+    // keep an earlier u_y seed
+    float *wuy = s->adj_wuy;
+    const int f0 = s->adj_uy_first, f1 = s->adj_uy_last;
+    s->adj_wuy = nullptr;
     s->adj_release();
+    s->adj_wuy = wuy;
+    s->adj_uy_first = f0;
+    s->adj_uy_last = f1;
     const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
     for (int k = 0; k < nck; ++k) {
         void *p = nullptr;
-        if (hipMalloc(&p, s->pop_bytes()) != hipSuccess) {
+        if (hipMalloc(&p, s->state_bytes()) != hipSuccess) {
             s->adj_release();
             return fail(MCG_E_NOMEM, "adjoint checkpoint allocation failed");
         }
@@ -821,8 +888,7 @@ int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every) {
     s->adj_step0 = s->step;
     for (int k = 0; k < nsteps; ++k) {
         if (k % checkpoint_every == 0)
-            HIP_TRY(hipMemcpyAsync(s->adj_ckpt[k / checkpoint_every], s->f[s->cur], s->pop_bytes(), hipMemcpyDeviceToDevice,
-                                   s->stream));
+            if (int rc = save_state(s, s->adj_ckpt[k / checkpoint_every])) return rc;
         const int rc = one_step(s);
         if (rc) return rc;
     }
@@ -833,17 +899,59 @@ int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every) {
     return MCG_OK;
 }
 
-// one step from population buffer src into dst (the solver's own buffers are
-// put back afterwards; the step counter advances)
-static int step_between(mcg_solver *s, void *src, void *dst) {
+int mcg_adjoint_set_uy_seed(mcg_solver *s, const float *w_uy, int first, int last) {
+    if (int rc = check_solver(s)) return rc;
+    if (!w_uy) {
+        if (s->adj_wuy) (void)hipFree(s->adj_wuy);
+        s->adj_wuy = nullptr;
+        s->adj_uy_first = 0;
+        s->adj_uy_last = -1;
+        return MCG_OK;
+    }
+    if (!s->jet()) return fail(MCG_E_UNSUPPORTED, "the u_y seed is defined for jet cases");
+    if (first < 0 || last < first) return fail(MCG_E_INVALID, "u_y seed steps must satisfy 0 <= first <= last");
+    if (!s->adj_wuy && hipMalloc((void **)&s->adj_wuy, s->N() * sizeof(float)) != hipSuccess)
+        return fail(MCG_E_NOMEM, "u_y seed allocation failed");
+    HIP_TRY(hipMemcpyAsync(s->adj_wuy, w_uy, s->N() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->adj_uy_first = first;
+    s->adj_uy_last = last;
+    return MCG_OK;
+}
+
+// One step from recorded state src into the population buffer dst_f (jet
+// cases: and the boundary arrays dst_bin, dst_bout, dst_gB), storing the
+// moments of src into rec (11 planes).  The solver's own buffers are put back
+// afterwards; the step counter advances.  Jet cases: the stale entries of the
+// inlet/outlet arrays (sources the boundaries never rewrite) are the same in
+// every state of one recorded forward, and dst must already hold them.
+static int step_from(mcg_solver *s, void *src, void *dst_f, float *dst_bin, float *dst_bout, float *dst_gB, float *rec) {
     void *f0 = s->f[0], *f1 = s->f[1];
+    float *bi[2] = {s->bin[0], s->bin[1]}, *bo[2] = {s->bout[0], s->bout[1]}, *gb[2] = {s->gB[0], s->gB[1]};
     const int cur = s->cur;
+    if (s->jet()) {
+        s->bin[0] = s->st_bin(src);
+        s->bin[1] = dst_bin;
+        s->bout[0] = s->st_bout(src);
+        s->bout[1] = dst_bout;
+        s->gB[0] = s->st_gB(src);
+        s->gB[1] = dst_gB;
+    }
     s->f[0] = src;
-    s->f[1] = dst;
+    s->f[1] = dst_f;
     s->cur = 0;
+    s->rec_mom = rec;
+    const bool mv = s->moments_valid;  // the solver's moment planes are not touched
     const int rc = one_step(s);
+    s->moments_valid = mv;
+    s->rec_mom = nullptr;
     s->f[0] = f0;
     s->f[1] = f1;
+    for (int b = 0; b < 2; ++b) {
+        s->bin[b] = bi[b];
+        s->bout[b] = bo[b];
+        s->gB[b] = gb[b];
+    }
     s->cur = cur;
     return rc;
 }
@@ -852,22 +960,73 @@ static int adj_alloc(mcg_solver *s) {
     auto get = [&](void **p, size_t nb) -> bool { return *p || hipMalloc(p, nb) == hipSuccess; };
     bool ok = true;
     for (int b = 0; b < 2; ++b) {
-        ok = ok && get(&s->adj_keep[b], s->pop_bytes());
         ok = ok && get((void **)&s->adj_lam[b], s->pop_bytes());
+        if (s->jet()) ok = ok && get((void **)&s->adj_lg[b], s->gB_bytes());
+    }
+    if (s->jet()) {
+        ok = ok && get((void **)&s->adj_din, 3 * s->nxz() * sizeof(double));
+        ok = ok && get((void **)&s->adj_psf, s->N() * sizeof(float));
+    } else {
+        ok = ok && get((void **)&s->adj_ps, size_t(s->P.plane) * sizeof(float));
     }
     ok = ok && get((void **)&s->adj_lm, size_t(11) * s->P.plane * sizeof(float));
     ok = ok && get((void **)&s->adj_lF, size_t(3) * s->P.plane * sizeof(float));
-    ok = ok && get((void **)&s->adj_ps, size_t(s->P.plane) * sizeof(float));
-    ok = ok && get((void **)&s->adj_acc, 2 * sizeof(double));
+    ok = ok && get((void **)&s->adj_acc, 4 * sizeof(double));
+    ok = ok && get((void **)&s->adj_mom, s->mom_set_bytes() * size_t(s->adj_every));
     const dim3 g = s->grid_interior();
     ok = ok && get((void **)&s->adj_part, 2 * sizeof(double) * size_t(g.x) * g.y * g.z);
     // a segment's first state is its checkpoint; the others are recomputed into these
     while (ok && int(s->adj_seg.size()) < s->adj_every - 1) {
         void *p = nullptr;
-        ok = hipMalloc(&p, s->pop_bytes()) == hipSuccess;
+        ok = hipMalloc(&p, s->state_bytes()) == hipSuccess;
         if (ok) s->adj_seg.push_back(p);
     }
     return ok ? MCG_OK : fail(MCG_E_NOMEM, "adjoint buffer allocation failed");
+}
+
+// adjoint of recorded step k (state st, its moments mom): lam_in -> lam_out;
+// jet cases also carry the outlet-state adjoint lg_in -> lg_out and add the
+// inlet gradients
+static int adjoint_step(mcg_solver *s, int k, void *st, const float *mom, float *lam_in, float *lam_out, float *lg_in,
+                        float *lg_out) {
+    const dim3 g = s->grid_interior();
+    AdjJet J{};
+    if (s->jet()) {
+        float *gBn = s->st_gB(st);
+        hipLaunchKernelGGL(k_psi_full, s->grid_full(), dim3(64, 4, 1), 0, s->stream, s->P, mom, gBn, s->grr, s->grb,
+                           s->adj_psf);
+        LAUNCH_CHECK("k_psi_full");
+        J.psf = s->adj_psf;
+        J.w_uy = (s->adj_wuy && k >= s->adj_uy_first && k <= s->adj_uy_last) ? s->adj_wuy : nullptr;
+        hipLaunchKernelGGL(k_adj_node<true>, g, dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_psf, lam_in, s->adj_lm,
+                           s->adj_lF, s->adj_part, J);
+        LAUNCH_CHECK("k_adj_node");
+        hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, size_t(g.x) * g.y * g.z,
+                           s->adj_acc);
+        LAUNCH_CHECK("k_adj_sum");
+        const int cols = (s->c.NX - 2) * (s->c.NZ - 2);
+        const dim3 gc(unsigned((cols + 255) / 256));
+        hipLaunchKernelGGL(k_adj_cols, gc, dim3(256), 0, s->stream, s->P, mom, lam_in, s->adj_lF, s->adj_lm, gBn, lg_in,
+                           lg_out, s->adj_din, s->adj_part, (unsigned long long)(s->adj_step0 + k));
+        LAUNCH_CHECK("k_adj_cols");
+        hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, size_t(gc.x), s->adj_acc + 2);
+        LAUNCH_CHECK("k_adj_sum");
+        hipLaunchKernelGGL(k_adj_pop<true>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_lm,
+                           s->adj_lF, lam_in, lam_out);
+    } else {
+        hipLaunchKernelGGL(k_psi_plane, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_ps);
+        LAUNCH_CHECK("k_psi_plane");
+        hipLaunchKernelGGL(k_adj_node<false>, g, dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_ps, lam_in, s->adj_lm,
+                           s->adj_lF, s->adj_part, J);
+        LAUNCH_CHECK("k_adj_node");
+        hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, size_t(g.x) * g.y * g.z,
+                           s->adj_acc);
+        LAUNCH_CHECK("k_adj_sum");
+        hipLaunchKernelGGL(k_adj_pop<false>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_lm,
+                           s->adj_lF, lam_in, lam_out);
+    }
+    LAUNCH_CHECK("k_adj_pop");
+    return MCG_OK;
 }
 
 int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb, float *out_lam_fr, float *out_lam_fb,
@@ -879,17 +1038,31 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     if (int rc = adj_alloc(s)) return rc;
     const size_t n = size_t(Q) * s->N();
     if (int rc = s->ensure_staging(2 * n * sizeof(float))) return rc;
-    // keep the final state (both buffers: the last moments come from the previous one)
+    // The current state (buffer cur0) is never written.  The other buffer
+    // (the previous state, which the next step overwrites) takes the output of
+    // each segment's last recomputed step, whose only use is the moments of
+    // the segment's last state; the solver's moments of the last step (the
+    // fields it exports) are taken from it first.
+    if (int rc = ensure_moments(s)) return rc;
     const int cur0 = s->cur;
-    for (int b = 0; b < 2; ++b)
-        HIP_TRY(hipMemcpyAsync(s->adj_keep[b], s->f[b], s->pop_bytes(), hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->staging, lam_fr, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->staging + n, lam_fb, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemsetAsync(s->adj_lam[0], 0, s->pop_bytes(), s->stream));
     hipLaunchKernelGGL(k_rows_from_ref, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->staging, s->staging + n,
                        s->adj_lam[0]);
     LAUNCH_CHECK("k_rows_from_ref");
-    HIP_TRY(hipMemsetAsync(s->adj_acc, 0, 2 * sizeof(double), s->stream));
+    HIP_TRY(hipMemsetAsync(s->adj_acc, 0, 4 * sizeof(double), s->stream));
+    if (s->jet()) {
+        // stale inlet/outlet entries, identical in every recorded state
+        for (void *p : s->adj_seg) {
+            HIP_TRY(hipMemcpyAsync(s->st_bin(p), s->st_bin(s->adj_ckpt[0]), s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+            HIP_TRY(hipMemcpyAsync(s->st_bout(p), s->st_bout(s->adj_ckpt[0]), s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+        }
+        // the objective does not read the final outlet state
+        HIP_TRY(hipMemsetAsync(s->adj_lg[0], 0, s->gB_bytes(), s->stream));
+        HIP_TRY(hipMemsetAsync(s->adj_lg[1], 0, s->gB_bytes(), s->stream));
+        HIP_TRY(hipMemsetAsync(s->adj_din, 0, 3 * s->nxz() * sizeof(double), s->stream));
+    }
     s->stats_frozen = true;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
@@ -898,28 +1071,24 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     int a = 0;
     int rc = MCG_OK;
     const int nck = int(s->adj_ckpt.size());
+    const size_t mset = s->mom_set_bytes() / sizeof(float);
     for (int seg = nck - 1; seg >= 0 && rc == MCG_OK; --seg) {
         const int n0 = seg * s->adj_every, n1 = std::min(n0 + s->adj_every, s->adj_nsteps);
-        // states n0 .. n1-1 of the segment: the checkpoint, then recomputed in place
+        // states n0 .. n1-1 of the segment: the checkpoint, then recomputed in
+        // place; every recomputed step records the moments of its source state
         auto state = [&](int k) -> void * { return k == n0 ? s->adj_ckpt[seg] : s->adj_seg[k - n0 - 1]; };
+        auto moms = [&](int k) -> float * { return s->adj_mom + size_t(k - n0) * mset; };
         s->step = s->adj_step0 + n0;
-        for (int k = n0; k + 1 < n1 && rc == MCG_OK; ++k) rc = step_between(s, state(k), state(k + 1));
+        for (int k = n0; k < n1 && rc == MCG_OK; ++k) {
+            if (k + 1 < n1) {
+                void *d = state(k + 1);
+                rc = step_from(s, state(k), d, s->st_bin(d), s->st_bout(d), s->st_gB(d), moms(k));
+            } else {
+                rc = step_from(s, state(k), s->f[cur0 ^ 1], s->bin[cur0 ^ 1], s->bout[cur0 ^ 1], s->gB[cur0 ^ 1], moms(k));
+            }
+        }
         for (int k = n1 - 1; k >= n0 && rc == MCG_OK; --k) {
-            hipLaunchKernelGGL(k_moments<MCG_STORE_FP32>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, state(k),
-                               s->bin[s->cur], s->bout[s->cur], s->mom);
-            LAUNCH_CHECK("k_moments (adjoint)");
-            hipLaunchKernelGGL(k_psi_plane, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_ps);
-            LAUNCH_CHECK("k_psi_plane");
-            const dim3 g = s->grid_interior();
-            hipLaunchKernelGGL(k_adj_node, g, dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_ps, s->adj_lam[a],
-                               s->adj_lm, s->adj_lF, s->adj_part);
-            LAUNCH_CHECK("k_adj_node");
-            hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, s->stream, s->adj_part, size_t(g.x) * g.y * g.z,
-                               s->adj_acc);
-            LAUNCH_CHECK("k_adj_sum");
-            hipLaunchKernelGGL(k_adj_pop, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_lm,
-                               s->adj_lF, s->adj_lam[a ^ 1]);
-            LAUNCH_CHECK("k_adj_pop");
+            rc = adjoint_step(s, k, state(k), moms(k), s->adj_lam[a], s->adj_lam[a ^ 1], s->adj_lg[a], s->adj_lg[a ^ 1]);
             a ^= 1;
         }
     }
@@ -931,6 +1100,8 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     s->adj_ms = double(ms);
+    s->cur = cur0;
+    s->step = s->adj_step0 + s->adj_nsteps;
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(s->staging, 0, 2 * n * sizeof(float), s->stream));
     hipLaunchKernelGGL(k_rows_to_ref, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->adj_lam[a], s->staging,
@@ -938,19 +1109,27 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     LAUNCH_CHECK("k_rows_to_ref");
     if (out_lam_fr) HIP_TRY(hipMemcpyAsync(out_lam_fr, s->staging, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     if (out_lam_fb) HIP_TRY(hipMemcpyAsync(out_lam_fb, s->staging + n, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-    double acc[2] = {0.0, 0.0};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     HIP_TRY(hipMemcpyAsync(acc, s->adj_acc, sizeof(acc), hipMemcpyDeviceToHost, s->stream));
-    // restore the final state
-    s->cur = cur0;
-    for (int b = 0; b < 2; ++b)
-        HIP_TRY(hipMemcpyAsync(s->f[b], s->adj_keep[b], s->pop_bytes(), hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    s->step = s->adj_step0 + s->adj_nsteps;
-    s->moments_valid = false;
+    s->adj_inlet[0] = acc[2];
+    s->adj_inlet[1] = acc[3];
     if (dparams) {
         dparams[0] = acc[0];
         dparams[1] = acc[1];
     }
+    return MCG_OK;
+}
+
+int mcg_adjoint_inlet_gradient(const mcg_solver *s, double *dparams, double *dinlet) {
+    if (int rc = check_solver(s)) return rc;
+    if (!s->jet()) return fail(MCG_E_UNSUPPORTED, "the inlet gradient is defined for jet cases");
+    if (!s->adj_din) return fail(MCG_E_STATE, "no adjoint sweep (call mcg_adjoint_backward first)");
+    if (dparams) {
+        dparams[0] = s->adj_inlet[0];
+        dparams[1] = s->adj_inlet[1];
+    }
+    if (dinlet) HIP_TRY(hipMemcpy(dinlet, s->adj_din, 3 * s->nxz() * sizeof(double), hipMemcpyDeviceToHost));
     return MCG_OK;
 }
 
@@ -972,6 +1151,7 @@ int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const doub
                     double *dparams) {
     if (!c || !fr0 || !fb0) return fail(MCG_E_INVALID, "null pointer");
     if (int rc = mcg_case_validate(c)) return rc;
+    if (c->y_mode != MCG_Y_PERIODIC) return fail(MCG_E_UNSUPPORTED, "jet cases: use mcg_cpu_adjoint_jet");
     if (int rc = adj_supported(*c, MCG_STORE_FP32)) return rc;
     if (c->sponge_K != 0.0) return fail(MCG_E_UNSUPPORTED, "the CPU adjoint assumes a uniform relaxation rate (sponge_K = 0)");
     if (nsteps < 0) return fail(MCG_E_INVALID, "nsteps < 0");

@@ -72,6 +72,7 @@ struct DevBufs {
     const float *ghost_rr;
     const float *ghost_rb;
     float *mom;            // 11 planes
+    float *mom_out;        // if set: the moments of every node of this step's source state (11 planes)
     double *stats;         // 7 planes of NY*NR (uy, uy2, ur, ur2, uruy, phi, phi2)
     unsigned long long *count;  // NY*NR
     int do_stats;

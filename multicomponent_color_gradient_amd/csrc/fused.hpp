@@ -220,7 +220,9 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // iterations only fill the ring and the moments of plane zb.  The gather and
 // the collision each appear once in the code, which keeps the loop inside the
 // instruction cache.
-template <bool STATS, int ST>
+// MOM: the core lanes also store the moments of their node (B.mom_out), the
+// variant the adjoint's recomputation runs.
+template <bool STATS, int ST, bool MOM = false>
 __global__ __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_per_eu(MCG_FUSED_WAVES_PER_EU)))
 void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     constexpr int TY = kFusedTY;
@@ -310,6 +312,12 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
             const DevParams PL = fresh_strides(P);
             const uint32_t p = pidx(PL, x, y, z);
             collide_store<ST>(n, B.f_dst, uint32_t(PL.PX), p);
+            if constexpr (MOM) {
+                const uint32_t q = midx(PL, x, y, z);
+                const size_t pl = PL.plane;
+                const float v[11] = {m.rr, m.rb, m.ux, m.uy, m.uz, m.pxx, m.pxy, m.pyy, m.pyz, m.pzz, m.pxz};
+                for (int k = 0; k < 11; ++k) __builtin_nontemporal_store(v[k], &B.mom_out[k * pl + q]);
+            }
             if (P.y_mode == MCG_Y_JET) {
                 if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z, B.step);
                 if (y == P.NY - 2) outlet_emit(P, B.gB_src, B.gB_dst, B.out_dst, m, x, z);
@@ -407,6 +415,8 @@ inline int launch_fused_st(const DevParams &P, const DevBufs &B, hipStream_t str
     const dim3 block(64, kFusedWaves, 1);
     if (B.do_stats) {
         hipLaunchKernelGGL((k_fused<true, ST>), dim3(G.nwg), block, fused_stats_lds(P), stream, P, B, G);
+    } else if (B.mom_out) {
+        hipLaunchKernelGGL((k_fused<false, ST, true>), dim3(G.nwg), block, 0, stream, P, B, G);
     } else {
         hipLaunchKernelGGL((k_fused<false, ST>), dim3(G.nwg), block, 0, stream, P, B, G);
     }

@@ -164,6 +164,23 @@ class Solver:
                                             as_ptr(out_b, f), as_ptr(dp, ctypes.c_double)))
         return out_r, out_b, (float(dp[0]), float(dp[1]))
 
+    def adjoint_set_uy_seed(self, w_uy, first: int = 0, last: int = -1) -> None:
+        """Jet cases: add sum over the moments passes of recorded steps first..last of
+        sum_x w_uy(x) u_y(x) to the objective (w_uy shape (NZ, NY, NX); None clears it)."""
+        if w_uy is None:
+            check(self.lib.mcg_adjoint_set_uy_seed(self._h, None, 0, -1))
+            return
+        w = np.ascontiguousarray(w_uy, dtype=np.float32).reshape(self.shape)
+        check(self.lib.mcg_adjoint_set_uy_seed(self._h, as_ptr(w, ctypes.c_float), int(first), int(last)))
+
+    def adjoint_inlet_gradient(self):
+        """Jet cases, after adjoint_backward: ((dJ/djet_velocity, dJ/dperturbation_amp),
+        dJ/de per column, shape (3, NZ, NX))."""
+        dp = np.zeros(2, np.float64)
+        din = np.zeros((3, self.case.NZ, self.case.NX), np.float64)
+        check(self.lib.mcg_adjoint_inlet_gradient(self._h, as_ptr(dp, ctypes.c_double), as_ptr(din, ctypes.c_double)))
+        return (float(dp[0]), float(dp[1])), din
+
     def adjoint_last_ms(self) -> float:
         """Device time of the last adjoint_backward sweep (recomputation + adjoint kernels)."""
         ms = ctypes.c_double()

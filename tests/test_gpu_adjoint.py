@@ -92,9 +92,19 @@ def test_backward_restores_state_and_statistics():
 
 def test_unsupported_and_misuse_errors():
     jet = m.jet_case(16, 16, 16, jet_radius=3.0)
-    s = m.Solver(jet)
+    s = m.Solver(jet, storage="fp16")
     s.init_jet()
-    with pytest.raises(m.McgError):
+    with pytest.raises(m.McgError):  # 16-bit populations
+        s.adjoint_record(2, 1)
+    jw = jet.copy()
+    jw.wrap_gradient = 1
+    s = m.Solver(jw)
+    s.init_jet()
+    with pytest.raises(m.McgError):  # jet with a wrapped gradient
+        s.adjoint_record(2, 1)
+    s = m.Solver(jet)
+    s.init_bubble()
+    with pytest.raises(m.McgError):  # first jet step after a bubble initialisation
         s.adjoint_record(2, 1)
     case, fr, fb = make_state(N=10)
     t = gpu_solver(case, fr, fb)

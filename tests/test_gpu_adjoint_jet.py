@@ -1,0 +1,102 @@
+"""GPU adjoint of jet steps (C ABI mcg_adjoint_record / mcg_adjoint_backward /
+mcg_adjoint_set_uy_seed / mcg_adjoint_inlet_gradient) against the FP64 CPU
+jet adjoint, which test_adjoint_jet_cpu.py pins to central finite differences
+of the FP64 oracle.  Same state and objective as there: the jet
+initialisation with the tanh inlet and the seeded transverse perturbation,
+interior populations perturbed at random, J = <lam, f_N> + sum over the
+moments passes of steps 2..5 of sum_x w(x) u_y(x).  FP32 forward and adjoint,
+FP64 parameter sums: relative agreement 1e-4 with the FP64 adjoint.
+"""
+import numpy as np
+import pytest
+
+import multicomponent_color_gradient_amd as m
+from multicomponent_color_gradient_amd.adjoint import cpu_adjoint_jet
+from test_adjoint_jet_cpu import INNER, STEPS, UY_STEPS, base_case, initial_state, weights
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.fixture(scope="module")
+def jet_setup():
+    case = base_case()
+    fr, fb, fields = initial_state(case)
+    # start from the FP32-rounded state so both adjoints linearise the same forward
+    fr, fb = fr.astype(np.float32).astype(np.float64), fb.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(1)
+    lam = [rng.standard_normal(fr.shape) * 0.1, rng.standard_normal(fb.shape) * 0.1]
+    for a in lam:
+        a[:, 0], a[:, -1] = 0, 0
+        a[:, :, 0], a[:, :, -1] = 0, 0
+        a[:, :, :, 0], a[:, :, :, -1] = 0, 0
+    w = weights(case)
+    ref = cpu_adjoint_jet(case, fr, fb, fields, STEPS, lam[0], lam[1], w_uy=w, uy_steps=UY_STEPS)
+    return case, fr, fb, lam, w, ref
+
+
+def run_gpu(case, fr, fb, lam, w, every, uy=UY_STEPS):
+    s = m.Solver(case)
+    s.init_jet()
+    s.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+    s.adjoint_set_uy_seed(w, *uy)
+    s.adjoint_record(STEPS, every)
+    gr, gb = s.populations()
+    lr, lb, (dA, dbeta) = s.adjoint_backward(lam[0], lam[1])
+    (dU, damp), din = s.adjoint_inlet_gradient()
+    return {"fr": gr, "fb": gb, "lam_fr": lr, "lam_fb": lb, "dA": dA, "dbeta": dbeta, "dU": dU, "damp": damp,
+            "dinlet": din}
+
+
+@pytest.mark.parametrize("every", [1, 3, STEPS])
+def test_gpu_jet_adjoint_matches_fp64_cpu_adjoint(jet_setup, every):
+    case, fr, fb, lam, w, ref = jet_setup
+    print(f"jet adjoint {case.NX}x{case.NY}x{case.NZ} steps={STEPS} every={every}", flush=True)
+    g = run_gpu(case, fr, fb, lam, w, every)
+    assert rel(g["fr"][INNER], ref["fr"][INNER]) < 1e-5
+    assert rel(g["fb"][INNER], ref["fb"][INNER]) < 1e-5
+    assert rel(g["lam_fr"][INNER], ref["lam_fr"][INNER]) < 1e-4
+    assert rel(g["lam_fb"][INNER], ref["lam_fb"][INNER]) < 1e-4
+    for k in ("dU", "damp", "dbeta"):
+        assert abs(g[k] - ref[k]) <= 1e-4 * abs(ref[k]), (k, g[k], ref[k])
+    # dJ/dA sums cancelling node terms: FP32 rounding shows at the 1e-3 level
+    assert abs(g["dA"] - ref["dA"]) <= 2e-3 * abs(ref["dA"]), (g["dA"], ref["dA"])
+    assert rel(g["dinlet"], ref["dinlet"]) < 1e-4
+    assert np.all(g["dinlet"][:, ref["dinlet"][1] == 0] == 0)
+    assert np.all(g["lam_fr"][:, 0] == 0) and np.all(g["lam_fb"][:, :, :, -1] == 0)  # ghost slots
+
+
+def test_gpu_jet_adjoint_without_seed_window(jet_setup):
+    """The u_y seed only acts on its steps: a window past the recorded steps
+    leaves <lam, f_N> alone, which the CPU adjoint gives with w = None."""
+    case, fr, fb, lam, w, _ = jet_setup
+    ref = cpu_adjoint_jet(case, fr, fb, initial_state(case)[2], STEPS, lam[0], lam[1])
+    g = run_gpu(case, fr, fb, lam, w, 2, uy=(STEPS, STEPS + 4))
+    assert rel(g["lam_fr"][INNER], ref["lam_fr"][INNER]) < 1e-4
+    assert abs(g["dU"] - ref["dU"]) <= 1e-4 * abs(ref["dU"])
+
+
+def test_gpu_jet_adjoint_restores_the_forward(jet_setup):
+    """After the backward sweep the solver continues from the recorded final
+    state: the populations, the inlet/outlet arrays and the outlet state."""
+    case, fr, fb, lam, w, _ = jet_setup
+    s = m.Solver(case)
+    s.init_jet()
+    s.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+    s.adjoint_record(STEPS, 3)
+    f0 = s.fields()
+    s.adjoint_backward(lam[0], lam[1])
+    f1 = s.fields()
+    for k in ("rhob", "uy"):
+        assert np.array_equal(f0[k], f1[k])
+    s.step(3)
+    t = m.Solver(case)
+    t.init_jet()
+    t.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+    t.step(STEPS + 3)
+    assert np.array_equal(s.populations()[0], t.populations()[0])
+    assert np.array_equal(s.populations()[1], t.populations()[1])
+    assert np.array_equal(s.fields()["rhob"], t.fields()["rhob"])
