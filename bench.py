@@ -179,7 +179,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-offset comparison run")
     ap.add_argument("--adjoint-steps", type=int, default=200)
-    ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--adjoint-every", type=int, default=200)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a device buffer of this size 3 times with hipMemcpy "
                          "(a plain copy kernel of known size that calibrates rocprofv3's FETCH_SIZE/WRITE_SIZE)")

@@ -225,9 +225,11 @@ int mcg_checkpoint_load(mcg_solver *s, const void *host_buf, uint64_t nbytes);
 
 /* Run nsteps forward steps from the current state and keep a copy of the
  * state (populations; jet cases also the inlet/outlet arrays) every
- * checkpoint_every steps (device memory: ceil(nsteps/K) + K + 3 population
- * sets during mcg_adjoint_backward).  Jet cases: right after an
- * initialisation the first step must follow mcg_init_jet. */
+ * checkpoint_every = K steps.  Device memory: ceil(nsteps/K) population sets
+ * for the checkpoints, and during mcg_adjoint_backward 4 population sets plus
+ * min(K, nsteps) moment sets of 11 floats per node (11/54 of a population
+ * set each).  Jet cases: right after an initialisation the first step must
+ * follow mcg_init_jet. */
 int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every);
 
 /* Jet cases: add sum over the moments passes of recorded steps first..last

@@ -96,8 +96,10 @@ struct mcg_solver {
     bool adj_recorded = false;
     int adj_nsteps = 0, adj_every = 0;
     int64_t adj_step0 = 0;
-    std::vector<void *> adj_ckpt, adj_seg;
+    std::vector<void *> adj_ckpt;
+    void *adj_pp[2] = {nullptr, nullptr};  // alternating states of the segment recomputation
     float *adj_mom = nullptr;  // moments of the states of one segment
+    float *adj_gbs = nullptr;  // jet cases: outlet states of the states of one segment
     float *adj_lam[2] = {nullptr, nullptr};
     float *adj_lm = nullptr, *adj_lF = nullptr, *adj_ps = nullptr;
     double *adj_acc = nullptr, *adj_part = nullptr;
@@ -113,14 +115,13 @@ struct mcg_solver {
 
     void adj_release() {
         for (void *p : adj_ckpt) (void)hipFree(p);
-        for (void *p : adj_seg) (void)hipFree(p);
         adj_ckpt.clear();
-        adj_seg.clear();
-        for (void *p : {(void *)adj_mom, (void *)adj_lam[0], (void *)adj_lam[1], (void *)adj_lm, (void *)adj_lF,
+        for (void *p : {adj_pp[0], adj_pp[1], (void *)adj_mom, (void *)adj_gbs, (void *)adj_lam[0], (void *)adj_lam[1], (void *)adj_lm, (void *)adj_lF,
                         (void *)adj_ps, (void *)adj_acc, (void *)adj_part, (void *)adj_lg[0], (void *)adj_lg[1],
                         (void *)adj_din, (void *)adj_psf, (void *)adj_wuy})
             if (p) (void)hipFree(p);
-        adj_mom = nullptr;
+        adj_mom = adj_gbs = nullptr;
+        adj_pp[0] = adj_pp[1] = nullptr;
         adj_lam[0] = adj_lam[1] = nullptr;
         adj_lm = adj_lF = adj_ps = nullptr;
         adj_acc = adj_part = nullptr;
@@ -139,6 +140,7 @@ struct mcg_solver {
     size_t bnd_bytes() const { return size_t(18) * nxz() * sizeof(float); }
     size_t gB_bytes() const { return size_t(5) * nxz() * sizeof(float); }
     size_t mom_set_bytes() const { return al256(size_t(11) * P.plane * sizeof(float)); }
+    size_t gB_set_bytes() const { return al256(gB_bytes()); }
     // one recorded forward state: the populations and, for jet cases, the
     // inlet and outlet source arrays and the outlet state (256-byte aligned parts)
     static size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -919,26 +921,32 @@ int mcg_adjoint_set_uy_seed(mcg_solver *s, const float *w_uy, int first, int las
     return MCG_OK;
 }
 
-// One step from recorded state src into the population buffer dst_f (jet
-// cases: and the boundary arrays dst_bin, dst_bout, dst_gB), storing the
-// moments of src into rec (11 planes).  The solver's own buffers are put back
-// afterwards; the step counter advances.  Jet cases: the stale entries of the
-// inlet/outlet arrays (sources the boundaries never rewrite) are the same in
-// every state of one recorded forward, and dst must already hold them.
-static int step_from(mcg_solver *s, void *src, void *dst_f, float *dst_bin, float *dst_bout, float *dst_gB, float *rec) {
+// One state of a recomputed segment: populations and, for jet cases, the
+// inlet/outlet source arrays and the outlet state.
+struct StateRef {
+    void *f;
+    float *bin, *bout, *gB;
+};
+
+// One step from state src into dst, storing the moments of src into rec (11
+// planes).  The solver's own buffers are put back afterwards; the step counter
+// advances.  Jet cases: the stale entries of the inlet/outlet arrays (sources
+// the boundaries never rewrite) are the same in every state of one recorded
+// forward, and dst must already hold them.
+static int step_from(mcg_solver *s, const StateRef &src, const StateRef &dst, float *rec) {
     void *f0 = s->f[0], *f1 = s->f[1];
     float *bi[2] = {s->bin[0], s->bin[1]}, *bo[2] = {s->bout[0], s->bout[1]}, *gb[2] = {s->gB[0], s->gB[1]};
     const int cur = s->cur;
     if (s->jet()) {
-        s->bin[0] = s->st_bin(src);
-        s->bin[1] = dst_bin;
-        s->bout[0] = s->st_bout(src);
-        s->bout[1] = dst_bout;
-        s->gB[0] = s->st_gB(src);
-        s->gB[1] = dst_gB;
+        s->bin[0] = src.bin;
+        s->bin[1] = dst.bin;
+        s->bout[0] = src.bout;
+        s->bout[1] = dst.bout;
+        s->gB[0] = src.gB;
+        s->gB[1] = dst.gB;
     }
-    s->f[0] = src;
-    s->f[1] = dst_f;
+    s->f[0] = src.f;
+    s->f[1] = dst.f;
     s->cur = 0;
     s->rec_mom = rec;
     const bool mv = s->moments_valid;  // the solver's moment planes are not touched
@@ -958,41 +966,37 @@ static int step_from(mcg_solver *s, void *src, void *dst_f, float *dst_bin, floa
 
 static int adj_alloc(mcg_solver *s) {
     auto get = [&](void **p, size_t nb) -> bool { return *p || hipMalloc(p, nb) == hipSuccess; };
+    const size_t seg_len = size_t(std::min(s->adj_every, s->adj_nsteps));  // states per segment
     bool ok = true;
     for (int b = 0; b < 2; ++b) {
         ok = ok && get((void **)&s->adj_lam[b], s->pop_bytes());
+        ok = ok && get(&s->adj_pp[b], s->state_bytes());
         if (s->jet()) ok = ok && get((void **)&s->adj_lg[b], s->gB_bytes());
     }
     if (s->jet()) {
         ok = ok && get((void **)&s->adj_din, 3 * s->nxz() * sizeof(double));
         ok = ok && get((void **)&s->adj_psf, s->N() * sizeof(float));
+        ok = ok && get((void **)&s->adj_gbs, s->gB_set_bytes() * seg_len);
     } else {
         ok = ok && get((void **)&s->adj_ps, size_t(s->P.plane) * sizeof(float));
     }
     ok = ok && get((void **)&s->adj_lm, size_t(11) * s->P.plane * sizeof(float));
     ok = ok && get((void **)&s->adj_lF, size_t(3) * s->P.plane * sizeof(float));
     ok = ok && get((void **)&s->adj_acc, 4 * sizeof(double));
-    ok = ok && get((void **)&s->adj_mom, s->mom_set_bytes() * size_t(s->adj_every));
+    ok = ok && get((void **)&s->adj_mom, s->mom_set_bytes() * seg_len);
     const dim3 g = s->grid_interior();
     ok = ok && get((void **)&s->adj_part, 2 * sizeof(double) * size_t(g.x) * g.y * g.z);
-    // a segment's first state is its checkpoint; the others are recomputed into these
-    while (ok && int(s->adj_seg.size()) < s->adj_every - 1) {
-        void *p = nullptr;
-        ok = hipMalloc(&p, s->state_bytes()) == hipSuccess;
-        if (ok) s->adj_seg.push_back(p);
-    }
     return ok ? MCG_OK : fail(MCG_E_NOMEM, "adjoint buffer allocation failed");
 }
 
-// adjoint of recorded step k (state st, its moments mom): lam_in -> lam_out;
-// jet cases also carry the outlet-state adjoint lg_in -> lg_out and add the
-// inlet gradients
-static int adjoint_step(mcg_solver *s, int k, void *st, const float *mom, float *lam_in, float *lam_out, float *lg_in,
-                        float *lg_out) {
+// adjoint of recorded step k (its moments mom; jet cases: its outlet state
+// gBn): lam_in -> lam_out; jet cases also carry the outlet-state adjoint
+// lg_in -> lg_out and add the inlet gradients
+static int adjoint_step(mcg_solver *s, int k, const float *gBn, const float *mom, float *lam_in, float *lam_out,
+                        float *lg_in, float *lg_out) {
     const dim3 g = s->grid_interior();
     AdjJet J{};
     if (s->jet()) {
-        float *gBn = s->st_gB(st);
         hipLaunchKernelGGL(k_psi_full, s->grid_full(), dim3(64, 4, 1), 0, s->stream, s->P, mom, gBn, s->grr, s->grb,
                            s->adj_psf);
         LAUNCH_CHECK("k_psi_full");
@@ -1054,7 +1058,7 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     HIP_TRY(hipMemsetAsync(s->adj_acc, 0, 4 * sizeof(double), s->stream));
     if (s->jet()) {
         // stale inlet/outlet entries, identical in every recorded state
-        for (void *p : s->adj_seg) {
+        for (void *p : s->adj_pp) {
             HIP_TRY(hipMemcpyAsync(s->st_bin(p), s->st_bin(s->adj_ckpt[0]), s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
             HIP_TRY(hipMemcpyAsync(s->st_bout(p), s->st_bout(s->adj_ckpt[0]), s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
         }
@@ -1071,24 +1075,33 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     int a = 0;
     int rc = MCG_OK;
     const int nck = int(s->adj_ckpt.size());
-    const size_t mset = s->mom_set_bytes() / sizeof(float);
+    const size_t mset = s->mom_set_bytes() / sizeof(float), gset = s->gB_set_bytes() / sizeof(float);
+    const bool jet = s->jet();
     for (int seg = nck - 1; seg >= 0 && rc == MCG_OK; --seg) {
         const int n0 = seg * s->adj_every, n1 = std::min(n0 + s->adj_every, s->adj_nsteps);
-        // states n0 .. n1-1 of the segment: the checkpoint, then recomputed in
-        // place; every recomputed step records the moments of its source state
-        auto state = [&](int k) -> void * { return k == n0 ? s->adj_ckpt[seg] : s->adj_seg[k - n0 - 1]; };
+        // Recompute the segment from its checkpoint through two alternating
+        // state buffers; each step records the moments (and, jet cases, the
+        // outlet state) of its source state, all the adjoint reads.  The last
+        // step's output goes to the solver's other buffer.
         auto moms = [&](int k) -> float * { return s->adj_mom + size_t(k - n0) * mset; };
+        auto gbs = [&](int k) -> float * { return jet ? s->adj_gbs + size_t(k - n0) * gset : nullptr; };
+        void *ck = s->adj_ckpt[seg];
+        StateRef src{ck, s->st_bin(ck), s->st_bout(ck), gbs(n0)};
+        if (jet) HIP_TRY(hipMemcpyAsync(gbs(n0), s->st_gB(ck), s->gB_bytes(), hipMemcpyDeviceToDevice, s->stream));
         s->step = s->adj_step0 + n0;
         for (int k = n0; k < n1 && rc == MCG_OK; ++k) {
+            StateRef dst;
             if (k + 1 < n1) {
-                void *d = state(k + 1);
-                rc = step_from(s, state(k), d, s->st_bin(d), s->st_bout(d), s->st_gB(d), moms(k));
+                void *p = s->adj_pp[(k - n0) & 1];
+                dst = StateRef{p, s->st_bin(p), s->st_bout(p), gbs(k + 1)};
             } else {
-                rc = step_from(s, state(k), s->f[cur0 ^ 1], s->bin[cur0 ^ 1], s->bout[cur0 ^ 1], s->gB[cur0 ^ 1], moms(k));
+                dst = StateRef{s->f[cur0 ^ 1], s->bin[cur0 ^ 1], s->bout[cur0 ^ 1], s->gB[cur0 ^ 1]};
             }
+            rc = step_from(s, src, dst, moms(k));
+            src = dst;
         }
         for (int k = n1 - 1; k >= n0 && rc == MCG_OK; --k) {
-            rc = adjoint_step(s, k, state(k), moms(k), s->adj_lam[a], s->adj_lam[a ^ 1], s->adj_lg[a], s->adj_lg[a ^ 1]);
+            rc = adjoint_step(s, k, gbs(k), moms(k), s->adj_lam[a], s->adj_lam[a ^ 1], s->adj_lg[a], s->adj_lg[a ^ 1]);
             a ^= 1;
         }
     }
