@@ -232,12 +232,28 @@ int mcg_checkpoint_load(mcg_solver *s, const void *host_buf, uint64_t nbytes);
  * follow mcg_init_jet. */
 int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every);
 
-/* Jet cases: add sum over the moments passes of recorded steps first..last
- * (0 = the first recorded step) of sum_x w_uy(x) u_y(x) to the objective of
- * mcg_adjoint_backward (w_uy: NX*NY*NZ floats, x fastest; NULL clears it).
- * Kept until cleared or mcg_adjoint_release.  No counterpart in the
- * reference (an extension). */
-int mcg_adjoint_set_uy_seed(mcg_solver *s, const float *w_uy, int first, int last);
+/* Jet cases: define the u_y functional J_uy = sum over the moments passes of
+ * steps first..last (solver step numbers: the pass of the step that takes
+ * step n to n+1 is pass n) of sum_x w_uy(x) u_y(x) (w_uy: NX*NY*NZ floats, x
+ * fastest; NULL clears it).  Ordinary steps from then on accumulate its value
+ * in FP64 (mcg_uy_functional), and mcg_adjoint_backward adds it to the
+ * objective: J = <lam, f_final> + J_uy.  Kept until cleared or the solver is
+ * destroyed; setting it restarts the sum.  No counterpart in the reference
+ * (an extension: the objective of the inverse jet problem). */
+int mcg_adjoint_set_uy_seed(mcg_solver *s, const float *w_uy, int64_t first, int64_t last);
+
+/* The value of the u_y functional accumulated by the steps taken since
+ * mcg_adjoint_set_uy_seed (FP64; synchronises the stream).  No counterpart in
+ * the reference (an extension). */
+int mcg_uy_functional(mcg_solver *s, double *value);
+
+/* Jet cases: add du to the inlet velocity of every jet column at every step
+ * (du: 3 x NX*NZ floats, component-major u_x, u_y, u_z planes, x fastest;
+ * entries of non-jet columns are ignored; NULL removes it).  The control of
+ * the inverse jet problem: mcg_adjoint_inlet_gradient returns dJ/d(du).  A
+ * solver setting, not part of a checkpoint.  No counterpart in the reference
+ * (an extension of jet/boundary_conditions.cuh:62-75). */
+int mcg_set_inlet_offset(mcg_solver *s, const float *du);
 
 /* Jet cases, after mcg_adjoint_backward: dparams = {dJ/djet_velocity,
  * dJ/dperturbation_amp} (FP64 sums) and dinlet = dJ/de for the inlet velocity
@@ -274,14 +290,15 @@ int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const doub
  * outlet are differentiated, the outlet state (rho_r, rho_b, u on the plane
  * NY-1) is carried between steps.  fields0 = rho_r, rho_b, u_x, u_y, u_z of the
  * initial state over the full grid (5 x NX*NY*NZ, ghosts included: they give
- * the constant ghost psi and the initial outlet state).  J = <lam, f_final> +
- * sum over the moments passes of steps uy_first..uy_last of sum_x w_uy(x) u_y(x)
- * (w_uy may be NULL).  dparams = {dJ/dA, dJ/dbeta, dJ/djet_velocity,
+ * the constant ghost psi and the initial outlet state); the steps are numbered
+ * from 0.  inlet_du (3 x NX*NZ, may be NULL) is added to the inlet velocity
+ * as by mcg_set_inlet_offset.  J = <lam, f_final> + sum over the moments
+ * passes of steps uy_first..uy_last of sum_x w_uy(x) u_y(x) (w_uy may be NULL).  dparams = {dJ/dA, dJ/dbeta, dJ/djet_velocity,
  * dJ/dperturbation_amp}; dinlet = dJ/de for the inlet velocity e of each jet
  * column summed over steps (3 x NX*NZ, component-major).  No counterpart in the
  * reference (an extension). */
 int mcg_cpu_adjoint_jet(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *fields0,
-                        const double *lam_fr, const double *lam_fb, const double *w_uy, int uy_first, int uy_last,
+                        const double *inlet_du, const double *lam_fr, const double *lam_fb, const double *w_uy, int uy_first, int uy_last,
                         double *out_fr, double *out_fb, double *out_lam_fr, double *out_lam_fb, double *dparams,
                         double *dinlet);
 

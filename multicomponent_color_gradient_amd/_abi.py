@@ -138,10 +138,13 @@ SIGNATURES = [
     ("mcg_adjoint_backward", c_int, [_PSOLVER, _F, _F, _F, _F, _D]),
     ("mcg_adjoint_last_ms", c_int, [_PSOLVER, _D]),
     ("mcg_adjoint_release", c_int, [_PSOLVER]),
-    ("mcg_adjoint_set_uy_seed", c_int, [_PSOLVER, _F, c_int, c_int]),
+    ("mcg_adjoint_set_uy_seed", c_int, [_PSOLVER, _F, c_int64, c_int64]),
     ("mcg_adjoint_inlet_gradient", c_int, [_PSOLVER, _D, _D]),
+    ("mcg_uy_functional", c_int, [_PSOLVER, _D]),
+    ("mcg_set_inlet_offset", c_int, [_PSOLVER, _F]),
     ("mcg_cpu_adjoint", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
-    ("mcg_cpu_adjoint_jet", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, c_int, c_int, _D, _D, _D, _D, _D, _D]),
+    ("mcg_cpu_adjoint_jet", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, _D, c_int, c_int, _D, _D, _D, _D, _D,
+                                    _D]),
     ("mcg_time_steps", c_int, [_PSOLVER, c_int, _D]),
     ("mcg_device_bytes", c_int, [_PSOLVER, POINTER(c_uint64)]),
     ("mcg_output_dirs", c_int, [POINTER(McgCase), c_char_p, c_char_p, c_char_p, c_char_p, c_size_t]),

@@ -48,14 +48,15 @@ def cpu_adjoint(case: McgCase, fr0: np.ndarray, fb0: np.ndarray, nsteps: int, la
 
 def cpu_adjoint_jet(case: McgCase, fr0: np.ndarray, fb0: np.ndarray, fields0: dict, nsteps: int,
                     lam_fr: np.ndarray | None = None, lam_fb: np.ndarray | None = None, w_uy: np.ndarray | None = None,
-                    uy_steps: tuple[int, int] = (0, -1)) -> dict:
+                    uy_steps: tuple[int, int] = (0, -1), inlet_du: np.ndarray | None = None) -> dict:
     """FP64 forward and adjoint of a jet case (inlet and convective outlet
     differentiated).  fields0: the initial fields dict (rhor, rhob, ux, uy, uz
-    over the full grid, ghosts included).  Returns the final state and, if a
+    over the full grid, ghosts included).  inlet_du: optional inlet velocity
+    offset per column, shape (3, NZ, NX).  Returns the final state and, if a
     seed is given, {"lam_fr", "lam_fb", "dA", "dbeta", "dU", "damp", "dinlet"}
     (dinlet: dJ/de per jet column summed over steps, shape (3, NZ, NX)).  J is
-    <lam, f_final> plus sum over the moments passes of steps uy_steps of
-    sum_x w_uy(x) u_y(x)."""
+    <lam, f_final> plus sum over the moments passes of steps uy_steps (numbered
+    from 0) of sum_x w_uy(x) u_y(x)."""
     lib = load_library()
     shape = (27, case.NZ, case.NY, case.NX)
     d = ctypes.c_double
@@ -74,8 +75,9 @@ def cpu_adjoint_jet(case: McgCase, fr0: np.ndarray, fb0: np.ndarray, fields0: di
     dinlet = np.zeros((3, case.NZ, case.NX))
     out["lam_fr"], out["lam_fb"] = np.empty(shape), np.empty(shape)
     c = case.copy()
+    du = None if inlet_du is None else np.ascontiguousarray(inlet_du, np.float64).reshape((3, case.NZ, case.NX))
     check(lib.mcg_cpu_adjoint_jet(ctypes.byref(c), int(nsteps), as_ptr(fr0, d), as_ptr(fb0, d), as_ptr(f0, d),
-                                  as_ptr(lam_fr, d) if seeded else None, as_ptr(lam_fb, d) if seeded else None,
+                                  None if du is None else as_ptr(du, d), as_ptr(lam_fr, d) if seeded else None, as_ptr(lam_fb, d) if seeded else None,
                                   None if w is None else as_ptr(w, d), int(uy_steps[0]), int(uy_steps[1]),
                                   as_ptr(out["fr"], d), as_ptr(out["fb"], d), as_ptr(out["lam_fr"], d),
                                   as_ptr(out["lam_fb"], d), as_ptr(dp, d), as_ptr(dinlet, d)))

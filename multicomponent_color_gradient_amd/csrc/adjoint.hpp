@@ -178,6 +178,27 @@ __global__ __launch_bounds__(kSumThreads) void k_adj_sum(const double *__restric
     }
 }
 
+// Per-workgroup partial sums of w(x) u_y(x) over the interior nodes (the
+// time-windowed u_y functional), part[2 b] (part[2 b + 1] = 0), for k_adj_sum.
+__global__ __launch_bounds__(256) void k_uy_partial(DevParams P, const float *__restrict__ mom, const float *__restrict__ w,
+                                                    double *part) {
+    const int x = 1 + int(blockIdx.x) * 64 + int(threadIdx.x);
+    const int y = 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
+    const int z = 1 + int(blockIdx.z);
+    double v = 0.0;
+    if (x <= P.NX - 2 && y <= P.NY - 2) v = double(w[gidx(P, x, y, z)]) * double(mom[3 * size_t(P.plane) + midx(P, x, y, z)]);
+    __shared__ double sh[4];
+    v = wave_sum(v);
+    const int wv = int(threadIdx.y);
+    if (threadIdx.x == 0) sh[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const size_t b = (size_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        part[2 * b] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        part[2 * b + 1] = 0.0;
+    }
+}
+
 // Colour-gradient transpose, psi derivative and moments-pass adjoint:
 // lambda^n at every interior node.  Jet cases: the transpose runs over the
 // interior neighbours without wrap, and the stale inlet/outlet slots (never
@@ -235,27 +256,23 @@ __global__ __launch_bounds__(256) void k_adj_pop(DevParams P, const float *__res
     }
 }
 
-// Inlet velocity of jet column (x,z) at step n with its derivatives with
-// respect to jet_velocity (dU) and perturbation_amp (damp); see inlet_velocity.
-__device__ __forceinline__ void inlet_velocity_grad(const DevParams &P, uint64_t step, int x, int z, float e[3], float dU[3],
-                                                    float damp[3]) {
-    float prof = 1.0f;
-    e[0] = e[2] = dU[0] = dU[2] = damp[0] = damp[1] = damp[2] = 0.0f;
+// Inlet velocity of jet column (x,z) at step n (inlet_velocity) with its
+// derivatives with respect to jet_velocity (dU) and perturbation_amp (damp).
+__device__ __forceinline__ void inlet_velocity_grad(const DevParams &P, const float *du, uint64_t step, int x, int z,
+                                                    float e[3], float dU[3], float damp[3]) {
+    inlet_velocity(P, du, step, x, z, e[0], e[1], e[2]);
+    dU[0] = dU[2] = damp[0] = damp[1] = damp[2] = 0.0f;
+    dU[1] = 1.0f;
     if (P.inlet_profile == MCG_INLET_TANH) {
         const float dx = float(x) - P.jet_x0, dz = float(z) - P.jet_z0;
-        prof = 0.5f * (1.0f - tanhf((sqrtf(dx * dx + dz * dz) - P.jet_R) / P.inlet_w));
+        dU[1] = 0.5f * (1.0f - tanhf((sqrtf(dx * dx + dz * dz) - P.jet_R) / P.inlet_w));
         const float sx = 2.0f * inlet_xi(P.pert_seed, step, x, z, P.NX, 1) - 1.0f;
         const float sz = 2.0f * inlet_xi(P.pert_seed, step, x, z, P.NX, 2) - 1.0f;
-        const float a = P.pert_amp * P.jet_U;
-        e[0] = a * sx;
-        e[2] = a * sz;
         dU[0] = P.pert_amp * sx;
         dU[2] = P.pert_amp * sz;
         damp[0] = P.jet_U * sx;
         damp[2] = P.jet_U * sz;
     }
-    e[1] = P.jet_U * prof;
-    dU[1] = prof;
 }
 
 // block sum of two doubles into part[2 b], part[2 b + 1] (256 threads)
@@ -290,7 +307,8 @@ __global__ __launch_bounds__(256) void k_adj_cols(DevParams P, const float *__re
                                                   const float *__restrict__ lam_in, const float *__restrict__ lam_F,
                                                   float *__restrict__ lam_m, const float *__restrict__ gBn,
                                                   const float *__restrict__ lgB_in, float *__restrict__ lgB_out,
-                                                  double *__restrict__ dinlet, double *part, unsigned long long step) {
+                                                  double *__restrict__ dinlet, double *part, unsigned long long step,
+                                                  const float *__restrict__ inlet_du) {
     const int c = int(blockIdx.x) * 256 + int(threadIdx.x);
     const int nx = P.NX - 2;
     const int x = 1 + c % nx, z = 1 + c / nx;
@@ -367,7 +385,7 @@ __global__ __launch_bounds__(256) void k_adj_cols(DevParams P, const float *__re
             const float rT = P.rhob0;
             const float oms = 1.0f - P.omega_b;
             float e[3], dU[3], da[3], le[3] = {0.f, 0.f, 0.f}, lP[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, lu[3] = {0.f, 0.f, 0.f};
-            inlet_velocity_grad(P, step, x, z, e, dU, da);
+            inlet_velocity_grad(P, inlet_du, step, x, z, e, dU, da);
             for (int k = 0; k < 9; ++k) {
                 const int i = kUp[k], tx = x + kCX[i], tz = z + kCZ[i];
                 if (tx < 1 || tx > P.NX - 2 || tz < 1 || tz > P.NZ - 2) continue;
@@ -575,6 +593,7 @@ struct CpuJetAdjoint {
     size_t N = 0;
     double A = 0, beta = 0, omega = 0;
     std::vector<double> gpsi;   // psi of every position at the initial state (ghosts constant)
+    std::vector<double> du;     // optional inlet velocity offset per column (3 x NX*NZ)
 
     size_t id(int x, int y, int z) const { return size_t(x) + size_t(NX) * (size_t(y) + size_t(NY) * size_t(z)); }
     static int wrap(int v, int n) { return v == 0 ? n - 2 : (v == n - 1 ? 1 : v); }
@@ -607,6 +626,10 @@ struct CpuJetAdjoint {
         }
         e[1] = c.jet_velocity * prof;
         dU[1] = prof;
+        if (!du.empty()) {
+            const size_t nxz = size_t(NX) * NZ, o = size_t(z) * NX + x;
+            for (int k = 0; k < 3; ++k) e[k] += du[k * nxz + o];
+        }
     }
 
     struct Step {

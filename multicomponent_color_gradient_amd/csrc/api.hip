@@ -93,6 +93,10 @@ struct mcg_solver {
     // the adjoint fields and the FP64 parameter sums
     bool stats_frozen = false;  // recomputation must not touch the statistics
     float *rec_mom = nullptr;   // if set, one_step stores the moments of its source state here
+    float *inlet_du = nullptr;  // jet cases: inlet velocity offset per column (3 x NX*NZ) or null
+    // the time-windowed u_y functional sum_{n=first..last} sum_x w(x) u_y(x, n)
+    // (FP64 accumulator and workgroup partials), accumulated by ordinary steps
+    double *obs_acc = nullptr, *obs_part = nullptr;
     bool adj_recorded = false;
     int adj_nsteps = 0, adj_every = 0;
     int64_t adj_step0 = 0;
@@ -110,7 +114,7 @@ struct mcg_solver {
     double *adj_din = nullptr;
     float *adj_psf = nullptr;  // psi over the full grid
     float *adj_wuy = nullptr;
-    int adj_uy_first = 0, adj_uy_last = -1;
+    int64_t adj_uy_first = 0, adj_uy_last = -1;  // solver step numbers
     double adj_inlet[2] = {0.0, 0.0};  // dJ/djet_velocity, dJ/dperturbation_amp of the last sweep
 
     void adj_release() {
@@ -118,7 +122,7 @@ struct mcg_solver {
         adj_ckpt.clear();
         for (void *p : {adj_pp[0], adj_pp[1], (void *)adj_mom, (void *)adj_gbs, (void *)adj_lam[0], (void *)adj_lam[1], (void *)adj_lm, (void *)adj_lF,
                         (void *)adj_ps, (void *)adj_acc, (void *)adj_part, (void *)adj_lg[0], (void *)adj_lg[1],
-                        (void *)adj_din, (void *)adj_psf, (void *)adj_wuy})
+                        (void *)adj_din, (void *)adj_psf})
             if (p) (void)hipFree(p);
         adj_mom = adj_gbs = nullptr;
         adj_pp[0] = adj_pp[1] = nullptr;
@@ -128,10 +132,16 @@ struct mcg_solver {
         adj_lg[0] = adj_lg[1] = nullptr;
         adj_din = nullptr;
         adj_psf = nullptr;
+        adj_recorded = false;
+    }
+    // the u_y functional (seed of the adjoint, observed by ordinary steps)
+    void uy_release() {
+        for (void *p : {(void *)adj_wuy, (void *)obs_acc, (void *)obs_part})
+            if (p) (void)hipFree(p);
         adj_wuy = nullptr;
+        obs_acc = obs_part = nullptr;
         adj_uy_first = 0;
         adj_uy_last = -1;
-        adj_recorded = false;
     }
 
     size_t N() const { return size_t(c.NX) * c.NY * c.NZ; }
@@ -168,6 +178,7 @@ struct mcg_solver {
         B.ghost_rb = grb;
         B.mom = mom;
         B.mom_out = rec_mom;
+        B.inlet_du = inlet_du;
         B.stats = stats;
         B.count = count;
         B.do_stats = do_stats ? 1 : 0;
@@ -337,6 +348,8 @@ int one_step(mcg_solver *s) {
     }
     const int64_t every = s->c.NSTATS_SAMPLE > 0 ? s->c.NSTATS_SAMPLE : 1;
     const bool do_stats = !s->stats_frozen && s->started && ((s->step - s->start_step) % every == 0);
+    // the u_y functional observes the moments pass of this step
+    const bool observe = s->adj_wuy && !s->stats_frozen && s->step >= s->adj_uy_first && s->step <= s->adj_uy_last;
     const DevBufs B = s->bufs(do_stats);
     if (s->kernel == MCG_KERNEL_TWOPASS) {
         int rc = launch_moments(s, s->cur);
@@ -353,7 +366,7 @@ int one_step(mcg_solver *s) {
     } else {
         DevBufs Bf = B;
         bool have_moments = false;
-        if (do_stats && !fused_has_stats(s->P, s->fused_lds)) {
+        if (do_stats && (observe || !fused_has_stats(s->P, s->fused_lds))) {
             // statistics from the moments of this step, LDS-privatised histogram
             int rc = launch_moments(s, s->cur);
             if (rc) return rc;
@@ -363,10 +376,21 @@ int one_step(mcg_solver *s) {
             LAUNCH_CHECK("k_radial");
             Bf.do_stats = 0;
             have_moments = true;
+        } else if (observe) {
+            Bf.mom_out = s->mom;  // the step stores the moments of its source state
+            have_moments = true;
         }
         const int rc = launch_fused(s->P, Bf, s->stream, s->num_cu);
         if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
         s->moments_valid = have_moments;
+    }
+    if (observe) {
+        const dim3 g = s->grid_interior();
+        hipLaunchKernelGGL(k_uy_partial, g, dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_wuy, s->obs_part);
+        LAUNCH_CHECK("k_uy_partial");
+        hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(kSumThreads), 0, s->stream, s->obs_part, size_t(g.x) * g.y * g.z,
+                           s->obs_acc);
+        LAUNCH_CHECK("k_adj_sum");
     }
     if (do_stats) ++s->samples;  // never during adjoint recomputation (stats_frozen)
     if (s->inlet_reset_pending) {
@@ -529,6 +553,8 @@ int mcg_destroy(mcg_solver *s) {
         if (p) (void)hipFree(p);
     s->release_staging();
     s->adj_release();
+    s->uy_release();
+    if (s->inlet_du) (void)hipFree(s->inlet_du);
     delete s;
     return MCG_OK;
 }
@@ -870,14 +896,7 @@ int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every) {
     // step's inlet reset keeps them only after the jet initialisation
     if (s->inlet_reset_pending && s->init_kind != kInitJet)
         return fail(MCG_E_UNSUPPORTED, "the adjoint of the first jet step needs the jet initialisation (take one step first)");
-    // keep an earlier u_y seed
-    float *wuy = s->adj_wuy;
-    const int f0 = s->adj_uy_first, f1 = s->adj_uy_last;
-    s->adj_wuy = nullptr;
     s->adj_release();
-    s->adj_wuy = wuy;
-    s->adj_uy_first = f0;
-    s->adj_uy_last = f1;
     const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
     for (int k = 0; k < nck; ++k) {
         void *p = nullptr;
@@ -901,23 +920,52 @@ int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every) {
     return MCG_OK;
 }
 
-int mcg_adjoint_set_uy_seed(mcg_solver *s, const float *w_uy, int first, int last) {
+int mcg_adjoint_set_uy_seed(mcg_solver *s, const float *w_uy, int64_t first, int64_t last) {
     if (int rc = check_solver(s)) return rc;
     if (!w_uy) {
-        if (s->adj_wuy) (void)hipFree(s->adj_wuy);
-        s->adj_wuy = nullptr;
-        s->adj_uy_first = 0;
-        s->adj_uy_last = -1;
+        s->uy_release();
         return MCG_OK;
     }
-    if (!s->jet()) return fail(MCG_E_UNSUPPORTED, "the u_y seed is defined for jet cases");
-    if (first < 0 || last < first) return fail(MCG_E_INVALID, "u_y seed steps must satisfy 0 <= first <= last");
-    if (!s->adj_wuy && hipMalloc((void **)&s->adj_wuy, s->N() * sizeof(float)) != hipSuccess)
-        return fail(MCG_E_NOMEM, "u_y seed allocation failed");
+    if (!s->jet()) return fail(MCG_E_UNSUPPORTED, "the u_y functional is defined for jet cases");
+    if (first < 0 || last < first) return fail(MCG_E_INVALID, "u_y window must satisfy 0 <= first <= last");
+    const dim3 g = s->grid_interior();
+    auto get = [&](void **p, size_t nb) -> bool { return *p || hipMalloc(p, nb) == hipSuccess; };
+    if (!get((void **)&s->adj_wuy, s->N() * sizeof(float)) || !get((void **)&s->obs_acc, 2 * sizeof(double)) ||
+        !get((void **)&s->obs_part, 2 * sizeof(double) * size_t(g.x) * g.y * g.z)) {
+        s->uy_release();
+        return fail(MCG_E_NOMEM, "u_y functional allocation failed");
+    }
     HIP_TRY(hipMemcpyAsync(s->adj_wuy, w_uy, s->N() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->obs_acc, 0, 2 * sizeof(double), s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->adj_uy_first = first;
     s->adj_uy_last = last;
+    return MCG_OK;
+}
+
+int mcg_uy_functional(mcg_solver *s, double *value) {
+    if (int rc = check_solver(s)) return rc;
+    if (!value) return fail(MCG_E_INVALID, "null pointer");
+    if (!s->obs_acc) return fail(MCG_E_STATE, "no u_y functional (call mcg_adjoint_set_uy_seed first)");
+    double acc[2];
+    HIP_TRY(hipMemcpyAsync(acc, s->obs_acc, sizeof(acc), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    *value = acc[0];
+    return MCG_OK;
+}
+
+int mcg_set_inlet_offset(mcg_solver *s, const float *du) {
+    if (int rc = check_solver(s)) return rc;
+    if (!du) {
+        if (s->inlet_du) (void)hipFree(s->inlet_du);
+        s->inlet_du = nullptr;
+        return MCG_OK;
+    }
+    if (!s->jet()) return fail(MCG_E_UNSUPPORTED, "the inlet offset is defined for jet cases");
+    if (!s->inlet_du && hipMalloc((void **)&s->inlet_du, 3 * s->nxz() * sizeof(float)) != hipSuccess)
+        return fail(MCG_E_NOMEM, "inlet offset allocation failed");
+    HIP_TRY(hipMemcpyAsync(s->inlet_du, du, 3 * s->nxz() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return MCG_OK;
 }
 
@@ -1001,7 +1049,8 @@ static int adjoint_step(mcg_solver *s, int k, const float *gBn, const float *mom
                            s->adj_psf);
         LAUNCH_CHECK("k_psi_full");
         J.psf = s->adj_psf;
-        J.w_uy = (s->adj_wuy && k >= s->adj_uy_first && k <= s->adj_uy_last) ? s->adj_wuy : nullptr;
+        const int64_t n = s->adj_step0 + k;
+        J.w_uy = (s->adj_wuy && n >= s->adj_uy_first && n <= s->adj_uy_last) ? s->adj_wuy : nullptr;
         hipLaunchKernelGGL(k_adj_node<true>, g, dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_psf, lam_in, s->adj_lm,
                            s->adj_lF, s->adj_part, J);
         LAUNCH_CHECK("k_adj_node");
@@ -1011,7 +1060,7 @@ static int adjoint_step(mcg_solver *s, int k, const float *gBn, const float *mom
         const int cols = (s->c.NX - 2) * (s->c.NZ - 2);
         const dim3 gc(unsigned((cols + 255) / 256));
         hipLaunchKernelGGL(k_adj_cols, gc, dim3(256), 0, s->stream, s->P, mom, lam_in, s->adj_lF, s->adj_lm, gBn, lg_in,
-                           lg_out, s->adj_din, s->adj_part, (unsigned long long)(s->adj_step0 + k));
+                           lg_out, s->adj_din, s->adj_part, (unsigned long long)(s->adj_step0 + k), s->inlet_du);
         LAUNCH_CHECK("k_adj_cols");
         hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, size_t(gc.x), s->adj_acc + 2);
         LAUNCH_CHECK("k_adj_sum");
@@ -1201,7 +1250,7 @@ int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const doub
 }
 
 int mcg_cpu_adjoint_jet(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *fields0,
-                        const double *lam_fr, const double *lam_fb, const double *w_uy, int uy_first, int uy_last,
+                        const double *inlet_du, const double *lam_fr, const double *lam_fb, const double *w_uy, int uy_first, int uy_last,
                         double *out_fr, double *out_fb, double *out_lam_fr, double *out_lam_fb, double *dparams,
                         double *dinlet) {
     if (!c || !fr0 || !fb0 || !fields0) return fail(MCG_E_INVALID, "null pointer");
@@ -1220,6 +1269,7 @@ int mcg_cpu_adjoint_jet(const mcg_case *c, int nsteps, const double *fr0, const 
     J.A = c->A_override > 0.0 ? c->A_override : c->sigma / (4.0 / 9.0) / c->taub;
     J.beta = c->beta_recolor;
     const size_t N = J.N, n = size_t(Q) * N, nxz = size_t(c->NX) * c->NZ;
+    if (inlet_du) J.du.assign(inlet_du, inlet_du + 3 * nxz);
     J.gpsi.resize(N);
     for (size_t g = 0; g < N; ++g) J.gpsi[g] = (fields0[g] - fields0[N + g]) / (fields0[g] + fields0[N + g]);
     std::vector<std::vector<double>> R(size_t(nsteps) + 1, std::vector<double>(n)), B(size_t(nsteps) + 1, std::vector<double>(n)),

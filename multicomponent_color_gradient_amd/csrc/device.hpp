@@ -73,6 +73,7 @@ struct DevBufs {
     const float *ghost_rb;
     float *mom;            // 11 planes
     float *mom_out;        // if set: the moments of every node of this step's source state (11 planes)
+    const float *inlet_du;  // if set: offset of the inlet velocity per column (3 x NX*NZ, component-major)
     double *stats;         // 7 planes of NY*NR (uy, uy2, ur, ur2, uruy, phi, phi2)
     unsigned long long *count;  // NY*NR
     int do_stats;
@@ -475,8 +476,8 @@ __device__ __forceinline__ float inlet_xi(uint32_t seed, uint64_t step, int x, i
 // (jet/boundary_conditions.cuh:62-75).  MCG_INLET_TANH (an extension beyond
 // the reference): u_y = U (1 - tanh((r - R)/w))/2 on the jet disk, and
 // transverse u_x, u_z = amp U (2 xi - 1) with xi drawn per step and column.
-__device__ __forceinline__ void inlet_velocity(const DevParams &P, uint64_t step, int x, int z, float &ux, float &uy,
-                                               float &uz) {
+__device__ __forceinline__ void inlet_velocity(const DevParams &P, const float *du, uint64_t step, int x, int z,
+                                               float &ux, float &uy, float &uz) {
     ux = 0.0f;
     uy = P.jet_U;
     uz = 0.0f;
@@ -489,12 +490,18 @@ __device__ __forceinline__ void inlet_velocity(const DevParams &P, uint64_t step
         ux = a * (2.0f * inlet_xi(P.pert_seed, step, x, z, P.NX, 1) - 1.0f);
         uz = a * (2.0f * inlet_xi(P.pert_seed, step, x, z, P.NX, 2) - 1.0f);
     }
+    if (du) {  // control of the inverse jet problem (an extension)
+        const size_t nxz = size_t(P.NX) * P.NZ, o = size_t(z) * P.NX + x;
+        ux += du[o];
+        uy += du[nxz + o];
+        uz += du[2 * nxz + o];
+    }
 }
 
 // Inlet populations sent by ghost node (x,0,z) of a jet column, computed from
 // the moments m of node (x,1,z): jet/boundary_conditions.cuh:97-148.
 __device__ __forceinline__ void inlet_emit(const DevParams &P, float *__restrict__ in_dst, const Moments &m, int x, int z,
-                                           uint64_t step) {
+                                           uint64_t step, const float *du) {
     const float rrB = (1.0f - 1.0f) * P.rhor0;  // jet node: is_jet = 1
     const float rbB = 1.0f * P.rhob0;
     const float rT = rrB + rbB;
@@ -504,7 +511,7 @@ __device__ __forceinline__ void inlet_emit(const DevParams &P, float *__restrict
     if (P.visc_mode == MCG_VISC_INTERFACE) om = 1.0f / tau_interface(P, rrB, rbB);
     const float oms = 1.0f - om;
     float ex, ey, ez;
-    inlet_velocity(P, step, x, z, ex, ey, ez);
+    inlet_velocity(P, du, step, x, z, ex, ey, ez);
     const size_t nxz = size_t(P.NX) * P.NZ;
     const size_t o = size_t(z) * P.NX + x;
     unroll<0, 9>([&](auto kc) {

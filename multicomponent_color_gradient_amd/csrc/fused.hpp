@@ -319,7 +319,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
                 for (int k = 0; k < 11; ++k) __builtin_nontemporal_store(v[k], &B.mom_out[k * pl + q]);
             }
             if (P.y_mode == MCG_Y_JET) {
-                if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z, B.step);
+                if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z, B.step, B.inlet_du);
                 if (y == P.NY - 2) outlet_emit(P, B.gB_src, B.gB_dst, B.out_dst, m, x, z);
             }
             if constexpr (STATS) {

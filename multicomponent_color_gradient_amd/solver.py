@@ -165,13 +165,29 @@ class Solver:
         return out_r, out_b, (float(dp[0]), float(dp[1]))
 
     def adjoint_set_uy_seed(self, w_uy, first: int = 0, last: int = -1) -> None:
-        """Jet cases: add sum over the moments passes of recorded steps first..last of
-        sum_x w_uy(x) u_y(x) to the objective (w_uy shape (NZ, NY, NX); None clears it)."""
+        """Jet cases: the u_y functional sum over the moments passes of steps first..last
+        (solver step numbers) of sum_x w_uy(x) u_y(x); steps accumulate it (uy_functional)
+        and adjoint_backward adds it to the objective (w_uy shape (NZ, NY, NX); None clears it)."""
         if w_uy is None:
             check(self.lib.mcg_adjoint_set_uy_seed(self._h, None, 0, -1))
             return
         w = np.ascontiguousarray(w_uy, dtype=np.float32).reshape(self.shape)
         check(self.lib.mcg_adjoint_set_uy_seed(self._h, as_ptr(w, ctypes.c_float), int(first), int(last)))
+
+    def uy_functional(self) -> float:
+        """Value of the u_y functional accumulated since adjoint_set_uy_seed (FP64)."""
+        v = ctypes.c_double()
+        check(self.lib.mcg_uy_functional(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_inlet_offset(self, du) -> None:
+        """Jet cases: offset added to the inlet velocity of every jet column, shape (3, NZ, NX)
+        (u_x, u_y, u_z planes); None removes it."""
+        if du is None:
+            check(self.lib.mcg_set_inlet_offset(self._h, None))
+            return
+        d = np.ascontiguousarray(du, dtype=np.float32).reshape((3, self.case.NZ, self.case.NX))
+        check(self.lib.mcg_set_inlet_offset(self._h, as_ptr(d, ctypes.c_float)))
 
     def adjoint_inlet_gradient(self):
         """Jet cases, after adjoint_backward: ((dJ/djet_velocity, dJ/dperturbation_amp),

@@ -169,6 +169,7 @@ struct OracleBase {
     virtual int writeset(uint8_t *cr, uint8_t *cb) = 0;
     virtual void gradient(int x, int y, int z, double F[3]) = 0;
     virtual int64_t get_step() = 0;
+    virtual void set_inlet_offset(const double *du) = 0;
 };
 
 template <class T>
@@ -183,6 +184,7 @@ struct Oracle final : OracleBase {
     std::vector<double> s_uy, s_uy2, s_ur, s_ur2, s_uruy, s_phi, s_phi2;
     std::vector<uint64_t> s_cnt;
     int64_t step_ = 0;
+    std::vector<double> inlet_du;  // optional per-column inlet velocity offset, 3 x NX*NZ
     bool started_ = false;
     int64_t start_step_ = 0, samples_ = 0;
     bool ready_ = false;
@@ -459,6 +461,21 @@ struct Oracle final : OracleBase {
             ex = a * (T(2) * T(inlet_xi(step_, x, z, 1)) - T(1));
             ez = a * (T(2) * T(inlet_xi(step_, x, z, 2)) - T(1));
         }
+        if (!inlet_du.empty()) {
+            const size_t nxz = size_t(NX) * NZ, o = size_t(z) * NX + x;
+            ex += T(inlet_du[o]);
+            ey += T(inlet_du[nxz + o]);
+            ez += T(inlet_du[2 * nxz + o]);
+        }
+    }
+    // Offset added to the inlet velocity of every jet column (the control of
+    // the inverse jet problem; an extension).  Null clears it.
+    void set_inlet_offset(const double *du) override {
+        if (!du) {
+            inlet_du.clear();
+            return;
+        }
+        inlet_du.assign(du, du + 3 * size_t(NX) * NZ);
     }
 
     // Inlet, jet/boundary_conditions.cuh:17-85 (inner (x,z) only, kernels.cu:98-114)
@@ -731,6 +748,7 @@ double mcgo_tke(void *h) { return static_cast<OracleBase *>(h)->tke(); }
 void mcgo_global_sums(void *h, double *o) { static_cast<OracleBase *>(h)->global_sums(o); }
 int mcgo_writeset(void *h, uint8_t *cr, uint8_t *cb) { return static_cast<OracleBase *>(h)->writeset(cr, cb); }
 void mcgo_gradient(void *h, int x, int y, int z, double *F) { static_cast<OracleBase *>(h)->gradient(x, y, z, F); }
+void mcgo_set_inlet_offset(void *h, const double *du) { static_cast<OracleBase *>(h)->set_inlet_offset(du); }
 
 // ---- single-node operators, exposed for the invariant tests (FP64) ----
 static const Lattice<double> &lat() {

@@ -39,6 +39,7 @@ _SIGS = [
     ("mcgo_global_sums", None, [c_void_p, _D]),
     ("mcgo_writeset", c_int, [c_void_p, POINTER(c_uint8), POINTER(c_uint8)]),
     ("mcgo_gradient", None, [c_void_p, c_int, c_int, c_int, _D]),
+    ("mcgo_set_inlet_offset", None, [c_void_p, _D]),
     ("mcgo_lattice", None, [POINTER(c_int), POINTER(c_int), POINTER(c_int), _D, _D, _D]),
     ("mcgo_feq", c_double, [c_int, c_double, c_double, c_double, c_double]),
     ("mcgo_fneqr", c_double, [c_int, _D, c_double, c_double, c_double]),
@@ -164,6 +165,14 @@ class Oracle:
         fr = np.ascontiguousarray(fr, np.float64).reshape((27,) + self.shape)
         fb = np.ascontiguousarray(fb, np.float64).reshape((27,) + self.shape)
         lib().mcgo_set_populations(self._h, _p(fr), _p(fb))
+
+    def set_inlet_offset(self, du):
+        """Offset added to the inlet velocity of every jet column, shape (3, NZ, NX); None clears it."""
+        if du is None:
+            lib().mcgo_set_inlet_offset(self._h, None)
+            return
+        self._du = np.ascontiguousarray(du, np.float64).reshape((3, self.case.NZ, self.case.NX))
+        lib().mcgo_set_inlet_offset(self._h, _p(self._du))
 
     def set_densities(self, rr, rb):
         rr = np.ascontiguousarray(rr, np.float64).reshape(self.shape)

@@ -43,13 +43,15 @@ def weights(case):
     return w
 
 
-def oracle_J(case, fr, fb, lam, w, steps=STEPS, **over):
+def oracle_J(case, fr, fb, lam, w, steps=STEPS, du=None, **over):
     c = case.copy()
     for k, v in over.items():
         setattr(c, k, v)
     o = Oracle(c)
     o.init_jet()
     o.set_populations(fr, fb)
+    if du is not None:
+        o.set_inlet_offset(du)
     J = 0.0
     for k in range(steps):
         o.step(1)
@@ -123,3 +125,45 @@ def test_inlet_gradient_sums_to_the_velocity_derivative(setup):
     prof = 0.5 * (1 - np.tanh((r - case.jet_radius) / case.interface_width))
     assert abs(float(np.sum(res["dinlet"][1] * prof)) - res["dU"]) <= 1e-10 * abs(res["dU"])
     assert np.all(res["dinlet"][:, r > case.jet_radius] == 0)
+
+
+def jet_columns(case):
+    z, x = np.meshgrid(np.arange(case.NZ), np.arange(case.NX), indexing="ij")
+    inner = (x >= 1) & (x <= case.NX - 2) & (z >= 1) & (z <= case.NZ - 2)
+    return inner & ((x - case.jet_x0) ** 2 + (z - case.jet_z0) ** 2 <= case.jet_radius ** 2)
+
+
+def test_inlet_gradient_matches_fd_on_random_directions(setup):
+    """dJ/d(du) for the per-column inlet velocity offset (the control of the
+    inverse jet problem) against central differences of the FP64 oracle on 8
+    seeded random directions over the jet columns, at a nonzero offset."""
+    case, fr, fb, lam, w, _ = setup
+    jet = jet_columns(case)
+    rng = np.random.default_rng(11)
+    du0 = rng.standard_normal((3, case.NZ, case.NX)) * 2e-3 * jet
+    res = cpu_adjoint_jet(case, fr, fb, initial_state(case)[2], STEPS, lam[0], lam[1], w_uy=w, uy_steps=UY_STEPS,
+                          inlet_du=du0)
+    assert np.all(res["dinlet"][:, ~jet] == 0)
+    for k in range(8):
+        d = rng.standard_normal((3, case.NZ, case.NX)) * jet
+        h = 1e-4 / np.max(np.abs(d))
+        jp, _ = oracle_J(case, fr, fb, lam, w, du=du0 + h * d)
+        jm, _ = oracle_J(case, fr, fb, lam, w, du=du0 - h * d)
+        fd = (jp - jm) / (2 * h)
+        adj = float(np.sum(res["dinlet"] * d))
+        assert abs(adj - fd) <= 2e-6 * abs(fd), (k, adj, fd)
+
+
+def test_uniform_axial_offset_equals_a_faster_jet(setup):
+    """Sharp inlet, no perturbation: an offset c on u_y of every jet column is
+    the same inlet as jet_velocity U + c (pins the offset in the oracle)."""
+    case0, fr, fb, lam, w, _ = setup
+    case = case0.copy()
+    case.inlet_profile = 0
+    case.perturbation_amp = 0.0
+    du = np.zeros((3, case.NZ, case.NX))
+    du[1][jet_columns(case)] = 0.01
+    ja, (ra, ba) = oracle_J(case, fr, fb, lam, w, du=du)
+    jb, (rb, bb) = oracle_J(case, fr, fb, lam, w, jet_velocity=case.jet_velocity + 0.01)
+    assert np.array_equal(ra, rb) and np.array_equal(ba, bb)
+    assert ja == jb

@@ -12,7 +12,7 @@ import pytest
 
 import multicomponent_color_gradient_amd as m
 from multicomponent_color_gradient_amd.adjoint import cpu_adjoint_jet
-from test_adjoint_jet_cpu import INNER, STEPS, UY_STEPS, base_case, initial_state, weights
+from test_adjoint_jet_cpu import INNER, STEPS, UY_STEPS, base_case, initial_state, jet_columns, oracle_J, weights
 
 pytestmark = pytest.mark.gpu
 
@@ -38,10 +38,12 @@ def jet_setup():
     return case, fr, fb, lam, w, ref
 
 
-def run_gpu(case, fr, fb, lam, w, every, uy=UY_STEPS):
+def run_gpu(case, fr, fb, lam, w, every, uy=UY_STEPS, du=None):
     s = m.Solver(case)
     s.init_jet()
     s.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+    if du is not None:
+        s.set_inlet_offset(du)
     s.adjoint_set_uy_seed(w, *uy)
     s.adjoint_record(STEPS, every)
     gr, gb = s.populations()
@@ -100,3 +102,49 @@ def test_gpu_jet_adjoint_restores_the_forward(jet_setup):
     assert np.array_equal(s.populations()[0], t.populations()[0])
     assert np.array_equal(s.populations()[1], t.populations()[1])
     assert np.array_equal(s.fields()["rhob"], t.fields()["rhob"])
+
+
+def offset(case, seed=11):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((3, case.NZ, case.NX)) * 2e-3 * jet_columns(case)).astype(np.float32).astype(np.float64)
+
+
+def test_gpu_inlet_offset_and_uy_functional_match_the_oracle(jet_setup):
+    """Forward with a per-column inlet velocity offset and the u_y functional
+    accumulated by the steps, against the FP64 oracle.  Populations to 1e-5;
+    the functional is a sum of velocities of both signs, so its FP32 error is
+    bounded against its scale sum|w| x (window steps) x U: 1e-5 of that."""
+    case, fr, fb, lam, w, _ = jet_setup
+    du = offset(case)
+    zero = [np.zeros_like(fr), np.zeros_like(fb)]
+    j_ref, (gr, gb) = oracle_J(case, fr, fb, zero, w, du=du)
+    s = m.Solver(case)
+    s.init_jet()
+    s.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+    s.set_inlet_offset(du)
+    s.adjoint_set_uy_seed(w, *UY_STEPS)
+    s.step(STEPS)
+    pr, pb = s.populations()
+    assert rel(pr[INNER], gr[INNER]) < 1e-5 and rel(pb[INNER], gb[INNER]) < 1e-5
+    scale = np.sum(np.abs(w)) * (UY_STEPS[1] - UY_STEPS[0] + 1) * case.jet_velocity
+    assert abs(s.uy_functional() - j_ref) <= 1e-5 * scale, (s.uy_functional(), j_ref)
+    # the two-pass kernel observes the same functional
+    t = m.Solver(case, kernel="twopass")
+    t.init_jet()
+    t.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+    t.set_inlet_offset(du)
+    t.adjoint_set_uy_seed(w, *UY_STEPS)
+    t.step(STEPS)
+    assert np.array_equal(t.populations()[0], pr)
+    assert t.uy_functional() == s.uy_functional()
+
+
+def test_gpu_jet_adjoint_with_inlet_offset(jet_setup):
+    case, fr, fb, lam, w, _ = jet_setup
+    du = offset(case)
+    ref = cpu_adjoint_jet(case, fr, fb, initial_state(case)[2], STEPS, lam[0], lam[1], w_uy=w, uy_steps=UY_STEPS,
+                          inlet_du=du)
+    g = run_gpu(case, fr, fb, lam, w, 4, du=du)
+    assert rel(g["lam_fb"][INNER], ref["lam_fb"][INNER]) < 1e-4
+    assert rel(g["dinlet"], ref["dinlet"]) < 1e-4
+    assert abs(g["dU"] - ref["dU"]) <= 1e-4 * abs(ref["dU"])
