@@ -1,0 +1,74 @@
+"""The inverse jet problem on a reduced grid: the GPU adjoint's gradient of the
+time-averaged centre-line u_y with respect to the per-column inlet velocity,
+against central finite differences of the FP64 oracle on 8 seeded random
+directions over the jet columns.
+
+Workload: the perturbed jet (tanh inlet, seeded 1% transverse perturbation)
+on 32x64x32 with R = 4 (52 jet columns), 500 steps, objective = mean of u_y on
+the centre line over y in [37, 50] during the last 125 steps: the full-size
+problem (128x256x128, R = 8, 2000 steps, y in [150, 200], last 500 steps)
+scaled by 1/4 in length and time.  FP32 forward and adjoint, FP64 gradient
+sums; FD in FP64 with h = 1e-4 (2e-3 U) on directions scaled to max |d| = 1.
+
+Tolerance: |adjoint - FD| <= 1e-4 x the largest |FD| over the 8 directions.
+The FP32 error is an absolute floor (about 5e-8 here, 3e-5 of the typical
+directional derivative); one direction's projection cancels to 1/13 of the
+others, so its own relative error is about 3.5e-4.
+"""
+import numpy as np
+import pytest
+
+from multicomponent_color_gradient_amd import inverse_jet as ij
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+STEPS, LAST, Y0, Y1 = 500, 125, 37, 50
+
+
+def small_case():
+    return ij.inverse_jet_case(32, 64, 32, jet_radius=4.0)
+
+
+def oracle_functional(case, du):
+    w = ij.centreline_weights(case, Y0, Y1, LAST).astype(np.float64)
+    o = Oracle(case)
+    o.init_jet()
+    o.set_inlet_offset(du)
+    J = 0.0
+    for n in range(STEPS):
+        o.step(1)  # after it the moment arrays hold the moments pass of step n
+        if n >= STEPS - LAST:
+            J += float(np.sum(w * o.fields()["uy"]))
+    o.close()
+    return J
+
+
+@pytest.fixture(scope="module")
+def grad():
+    case = small_case()
+    print(f"inverse jet {case.NX}x{case.NY}x{case.NZ} steps={STEPS}", flush=True)
+    return case, ij.gradient(case, STEPS, LAST, Y0, Y1, checkpoint_every=100)
+
+
+def test_functional_matches_the_oracle(grad):
+    case, g = grad
+    j_ref = oracle_functional(case, np.zeros((3, case.NZ, case.NX)))
+    assert abs(g["J"] - j_ref) <= 1e-5 * case.jet_velocity, (g["J"], j_ref)
+    assert g["J"] == ij.functional(case, STEPS, LAST, Y0, Y1)  # the plain forward accumulates the same sum
+
+
+def test_inlet_gradient_matches_fp64_fd_on_8_random_directions(grad):
+    case, g = grad
+    jet = ij.jet_mask(case)
+    assert jet.sum() == 52
+    assert np.all(g["dinlet"][:, ~jet] == 0)
+    h = 1e-4
+    adj, fd = [], []
+    for k, d in enumerate(ij.random_directions(case, 8, seed=5)):
+        fd.append((oracle_functional(case, h * d) - oracle_functional(case, -h * d)) / (2 * h))
+        adj.append(float(np.sum(g["dinlet"] * d)))
+        print(f"direction {k}: adjoint {adj[-1]:.9e} fd {fd[-1]:.9e} rel {abs(adj[-1] - fd[-1]) / abs(fd[-1]):.2e}",
+              flush=True)
+    adj, fd = np.array(adj), np.array(fd)
+    assert np.max(np.abs(adj - fd)) <= 1e-4 * np.max(np.abs(fd)), (adj, fd)
