@@ -133,6 +133,62 @@ def make_steps_function():
     return LBMSteps
 
 
+def make_jet_steps_function():
+    """``LBMJetSteps.apply(fr0, fb0, A, beta, U, amp, du, case, nsteps, checkpoint_every, w_uy, window)``
+    -> (fr, fb, J_uy): nsteps GPU steps of a jet case from the jet initialisation
+    with populations (fr0, fb0), A_override, beta_recolor, jet_velocity and
+    perturbation_amp replaced by the tensors A, beta, U, amp, and the inlet
+    velocity offset du (3, NZ, NX) added at every jet column.  J_uy is the u_y
+    functional sum over the moments passes of steps window[0]..window[1] of
+    sum_x w_uy(x) u_y(x) (0 when w_uy is None).  backward runs the GPU jet
+    adjoint and returns gradients for fr0, fb0, A, beta, U, amp and du."""
+    torch = _torch()
+
+    class LBMJetSteps(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, fr0, fb0, A, beta, U, amp, du, case, nsteps, checkpoint_every, w_uy, window):
+            c = case.copy()
+            c.A_override = float(A)
+            c.beta_recolor = float(beta)
+            c.jet_velocity = float(U)
+            c.perturbation_amp = float(amp)
+            s = Solver(c)
+            s.init_jet()
+            s.set_populations(fr0.detach().cpu().numpy(), fb0.detach().cpu().numpy())
+            s.set_inlet_offset(du.detach().cpu().numpy())
+            w = None if w_uy is None else np.ascontiguousarray(w_uy.detach().cpu().numpy(), np.float32)
+            if w is not None:
+                s.adjoint_set_uy_seed(w, int(window[0]), int(window[1]))
+            s.adjoint_record(nsteps, checkpoint_every)
+            J = s.uy_functional() if w is not None else 0.0
+            fr, fb = s.populations()
+            ctx.solver, ctx.w, ctx.window = s, w, window
+            ctx.device, ctx.dtype = fr0.device, fr0.dtype
+            return (torch.from_numpy(fr).to(fr0.device), torch.from_numpy(fb).to(fr0.device),
+                    torch.tensor(J, dtype=torch.float64))
+
+        @staticmethod
+        def backward(ctx, g_fr, g_fb, g_J):
+            s = ctx.solver
+            if g_fr is None:
+                g_fr = torch.zeros((27,) + s.shape)
+            if g_fb is None:
+                g_fb = torch.zeros((27,) + s.shape)
+            if ctx.w is not None:
+                gj = 0.0 if g_J is None else float(g_J)
+                s.adjoint_set_uy_seed(ctx.w * np.float32(gj), int(ctx.window[0]), int(ctx.window[1]))
+            lr, lb, (dA, dbeta) = s.adjoint_backward(g_fr.detach().cpu().numpy(), g_fb.detach().cpu().numpy())
+            (dU, damp), din = s.adjoint_inlet_gradient()
+            s.close()
+            ctx.solver = None
+            t = lambda a: torch.from_numpy(a).to(device=ctx.device, dtype=ctx.dtype)  # noqa: E731
+            d64 = lambda v: torch.tensor(v, dtype=torch.float64)  # noqa: E731
+            return (t(lr), t(lb), d64(dA), d64(dbeta), d64(dU), d64(damp), torch.from_numpy(din),
+                    None, None, None, None, None)
+
+    return LBMJetSteps
+
+
 def bubble_centroid_height(fr: "torch.Tensor", fb: "torch.Tensor") -> "torch.Tensor":
     """Height (y) of the blue component's centroid over the interior nodes: the
     objective of the rising-bubble adjoint (differentiable in torch)."""

@@ -148,3 +148,38 @@ def test_gpu_jet_adjoint_with_inlet_offset(jet_setup):
     assert rel(g["lam_fb"][INNER], ref["lam_fb"][INNER]) < 1e-4
     assert rel(g["dinlet"], ref["dinlet"]) < 1e-4
     assert abs(g["dU"] - ref["dU"]) <= 1e-4 * abs(ref["dU"])
+
+
+def test_torch_autograd_jet_matches_the_abi(jet_setup):
+    """LBMJetSteps through torch.autograd: J = <lam, f_N> + 2 J_uy; its
+    gradients equal the direct C-ABI adjoint with the u_y seed scaled by 2,
+    bit for bit, and J_uy the functional of a plain forward."""
+    import torch
+
+    from multicomponent_color_gradient_amd.adjoint import make_jet_steps_function
+
+    case, fr, fb, lam, w, _ = jet_setup
+    du = offset(case)
+    F = make_jet_steps_function()
+    fr0 = torch.tensor(fr, dtype=torch.float32, requires_grad=True)
+    fb0 = torch.tensor(fb, dtype=torch.float32, requires_grad=True)
+    A = torch.tensor(case.A_override, dtype=torch.float64, requires_grad=True)
+    beta = torch.tensor(case.beta_recolor, dtype=torch.float64, requires_grad=True)
+    U = torch.tensor(case.jet_velocity, dtype=torch.float64, requires_grad=True)
+    amp = torch.tensor(case.perturbation_amp, dtype=torch.float64, requires_grad=True)
+    dut = torch.tensor(du, dtype=torch.float64, requires_grad=True)
+    gr, gb, juy = F.apply(fr0, fb0, A, beta, U, amp, dut, case, STEPS, 3, torch.tensor(w), UY_STEPS)
+    J = (torch.tensor(lam[0], dtype=torch.float32) * gr).sum() + (torch.tensor(lam[1], dtype=torch.float32) * gb).sum()
+    (J.double() + 2.0 * juy).backward()
+    g = run_gpu(case, fr, fb, lam, 2.0 * w, 3, du=du)
+    assert np.array_equal(fr0.grad.numpy(), g["lam_fr"]) and np.array_equal(fb0.grad.numpy(), g["lam_fb"])
+    assert float(A.grad) == g["dA"] and float(beta.grad) == g["dbeta"]
+    assert float(U.grad) == g["dU"] and float(amp.grad) == g["damp"]
+    assert np.array_equal(dut.grad.numpy(), g["dinlet"])
+    s = m.Solver(case)
+    s.init_jet()
+    s.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+    s.set_inlet_offset(du)
+    s.adjoint_set_uy_seed(w, *UY_STEPS)
+    s.step(STEPS)
+    assert float(juy.detach()) == s.uy_functional()
