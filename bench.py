@@ -126,7 +126,7 @@ def adjoint_jet_ratio(steps: int, every: int) -> dict:
     fwd_ms = s.time_steps(steps)
     w = np.zeros(s.shape, np.float32)
     w[int(c.jet_z0), 1:-1, int(c.jet_x0)] = 1.0
-    n0 = s.step_count()
+    n0 = s.step_count
     s.adjoint_set_uy_seed(w, n0, n0 + steps - 1)
     s.adjoint_record(steps, every)
     lam = np.zeros((27,) + s.shape, np.float32)

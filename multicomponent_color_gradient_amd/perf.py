@@ -4,7 +4,9 @@ Reads the files ``rocprofv3 --output-format csv`` writes:
   * ``*_kernel_stats.csv`` (``--kernel-trace --stats``): per-kernel call
     count and durations;
   * ``*_counter_collection.csv`` (``--pmc FETCH_SIZE`` or ``--pmc WRITE_SIZE``):
-    one row per dispatch and counter, values in KiB.
+    one row per dispatch and counter, values in KiB;
+  * ``*_kernel_trace.csv`` (``--kernel-trace``): one row per dispatch with its
+    grid, which separates the launches of one kernel on different grids.
 and turns them into the figures the benchmark reports: bytes per launch
 (scaled by a plain copy of known size, see ``copy_probe_scale``), bytes per
 lattice update, GB/s and the fraction of the HBM peak.  MLUPS follows
@@ -12,6 +14,7 @@ utilities/countmlups.cuh:42-48 (steps * NX*NY*NZ / t / 1e6).
 
 Command line:
   python -m multicomponent_color_gradient_amd.perf stats <kernel_stats.csv>
+  python -m multicomponent_color_gradient_amd.perf trace <kernel_trace.csv>
   python -m multicomponent_color_gradient_amd.perf traffic <fetch.csv> <write.csv> --kernel k_fused --nodes N
 """
 from __future__ import annotations
@@ -46,6 +49,35 @@ def read_kernel_stats(path: str) -> list[dict]:
                         "avg_ns": float(r["AverageNs"]), "min_ns": float(r["MinNs"]), "max_ns": float(r["MaxNs"]),
                         "percent": float(r["Percentage"])})
     return out
+
+
+def trace_stats(path: str) -> list[dict]:
+    """Per (kernel, grid size in work-items) from a kernel_trace.csv: calls and
+    average / min / max duration in ns, in order of first dispatch."""
+    groups: dict = {}
+    with open(path, newline="") as fh:
+        for r in csv.DictReader(fh):
+            grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+            d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            key = (r["Kernel_Name"], grid)
+            g = groups.setdefault(key, {"name": key[0], "grid": grid, "first": int(r["Dispatch_Id"]), "calls": 0,
+                                        "total_ns": 0.0, "min_ns": float("inf"), "max_ns": 0.0})
+            g["calls"] += 1
+            g["total_ns"] += d
+            g["min_ns"] = min(g["min_ns"], d)
+            g["max_ns"] = max(g["max_ns"], d)
+    out = sorted(groups.values(), key=lambda g: g["first"])
+    for g in out:
+        g["avg_ns"] = g["total_ns"] / g["calls"]
+    return out
+
+
+def trace_table(rows: list[dict]) -> str:
+    lines = ["| kernel | grid (work-items) | calls | avg us | min us | max us |", "|---|---|---|---|---|---|"]
+    for r in rows:
+        lines.append(f"| `{r['name'][:70]}` | {r['grid']} | {r['calls']} | {r['avg_ns'] / 1e3:.1f} | "
+                     f"{r['min_ns'] / 1e3:.1f} | {r['max_ns'] / 1e3:.1f} |")
+    return "\n".join(lines)
 
 
 def read_counters(path: str) -> dict:
@@ -111,6 +143,8 @@ def main(argv=None) -> int:
     sub = ap.add_subparsers(dest="cmd", required=True)
     a = sub.add_parser("stats")
     a.add_argument("csv")
+    t = sub.add_parser("trace")
+    t.add_argument("csv")
     b = sub.add_parser("traffic")
     b.add_argument("fetch_csv")
     b.add_argument("write_csv")
@@ -119,6 +153,8 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats_table(read_kernel_stats(args.csv)))
+    elif args.cmd == "trace":
+        print(trace_table(trace_stats(args.csv)))
     else:
         t = traffic(read_counters(args.fetch_csv), read_counters(args.write_csv), args.kernel, args.nodes)
         for k, v in t.items():

@@ -83,3 +83,23 @@ def test_cli(tmp_path, capsys):
     fp, wp = synthetic(tmp_path)
     assert perf.main(["traffic", fp, wp, "--kernel", "k_fused", "--nodes", "10"]) == 0
     assert "bytes_per_launch" in capsys.readouterr().out
+
+
+def test_trace_groups_launches_by_grid(tmp_path):
+    cols = ["Kind", "Agent_Id", "Queue_Id", "Stream_Id", "Thread_Id", "Dispatch_Id", "Kernel_Id", "Kernel_Name",
+            "Correlation_Id", "Start_Timestamp", "End_Timestamp", "LDS_Block_Size", "Scratch_Size", "VGPR_Count",
+            "Accum_VGPR_Count", "SGPR_Count", "Workgroup_Size_X", "Workgroup_Size_Y", "Workgroup_Size_Z",
+            "Grid_Size_X", "Grid_Size_Y", "Grid_Size_Z"]
+    p = tmp_path / "run_kernel_trace.csv"
+    with open(p, "w", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow(cols)
+        # k on a big grid twice (1000 ns, 3000 ns), then on a small grid once, then another kernel
+        for disp, name, t0, t1, gx in [(1, "k", 0, 1000, 4096), (2, "k", 5000, 8000, 4096), (3, "k", 9000, 9100, 64),
+                                       (4, "q", 9200, 9300, 64)]:
+            w.writerow(["KERNEL_DISPATCH", "Agent 1", 1, 0, 1, disp, 1, name, disp, t0, t1, 0, 0, 32, 0, 32, 64, 4, 1,
+                        gx, 4, 2])
+    rows = perf.trace_stats(str(p))
+    assert [(r["name"], r["grid"], r["calls"]) for r in rows] == [("k", 32768, 2), ("k", 512, 1), ("q", 512, 1)]
+    assert rows[0]["avg_ns"] == 2000 and rows[0]["min_ns"] == 1000 and rows[0]["max_ns"] == 3000
+    assert "| `k` | 32768 | 2 | 2.0 | 1.0 | 3.0 |" in perf.trace_table(rows)
