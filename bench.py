@@ -138,6 +138,43 @@ def adjoint_jet_ratio(steps: int, every: int) -> dict:
             "ratio": round(bwd_ms / fwd_ms, 3)}
 
 
+def interface_rich(steps: int, warmup: int, kernel: str) -> dict:
+    """MLUPS on the interface-rich workload: the 256x512x256 jet grid and
+    parameters, initialised with 256 seeded tanh spheres (seed 7, R in [6, 14],
+    width 2) so that a large share of the nodes takes the interface branch of
+    the collision (the worst ALU mix), FP32 and FP16-offset storage."""
+    import numpy as np
+
+    import multicomponent_color_gradient_amd as m
+
+    c = m.perturbed_jet_case(256, 512, 256, jet_radius=16.0)
+    c.NSTATS_START_STEP = 0
+    c.NSTATS_SAMPLE = 10
+    out = {"workload": "jet grid 256x512x256 initialised with 256 tanh spheres (seed 7, R in [6,14], width 2)"}
+    cells, interior = c.NX * c.NY * c.NZ, m.interior_nodes(c)
+    for storage in ("fp32", "fp16"):
+        s = m.Solver(c, storage=storage, kernel=kernel)
+        s.init_spheres(256, 7, 6.0, 14.0)
+        print(f"bench: interface-rich 256x512x256 storage={storage} warmup={warmup} steps={steps}", file=sys.stderr,
+              flush=True)
+        s.step(warmup)
+        ms = s.time_steps(steps)
+        s.check_health()
+        if storage == "fp32":
+            # share of the interior nodes on the interface branch (4 rho_r rho_b / rho^2 > 1e-4,
+            # lbm.cuh:214-216) in the moments of the last timed step
+            f = s.fields()
+            rr, rb = f["rhor"][1:-1, 1:-1, 1:-1], f["rhob"][1:-1, 1:-1, 1:-1]
+            out["interface_fraction"] = round(float(np.mean(4 * rr * rb / (rr + rb) ** 2 > 1e-4)), 4)
+            del f, rr, rb
+        s.close()
+        avg = ms / steps
+        gbs = BYTES_PER_LU[storage] * interior / (avg / 1e3) / 1e9
+        out[storage] = {"mlups": round(steps * cells / (ms / 1e3) / 1e6, 2), "ms_per_step": round(avg, 5),
+                        "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def cpu_baseline(case, init: str, budget_s: float):
     """FP64 OpenMP oracle on the same grid: 1 warm-up step, then steps until budget_s."""
     from oracle import oracle as orc
@@ -179,6 +216,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-offset comparison run")
+    ap.add_argument("--no-interface-rich", action="store_true", help="skip the interface-rich (256 spheres) run")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=200)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -255,6 +293,8 @@ def main():
                               "roofline": {"achieved": round(r16["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                            "frac": round(r16["achieved"] / HBM_PEAK_GBS, 4), "traffic": r16["traffic"],
                                            "bytes_per_lu": r16["bpl"]}}
+    if not args.no_interface_rich and args.storage == "fp32" and args.workload == "jet256":
+        out["interface_rich"] = interface_rich(args.steps, args.warmup, args.kernel)
     if not args.no_adjoint and args.storage == "fp32":
         out["adjoint"] = adjoint_ratio(args.adjoint_steps, args.adjoint_every)
         out["adjoint_jet"] = adjoint_jet_ratio(args.adjoint_steps, args.adjoint_every)

@@ -144,6 +144,18 @@ int mcg_init_bubble(mcg_solver *s);
  * (1 + tanh((R - r)/interface_width))/2, populations w_i * rho_k. */
 int mcg_init_bubble_tanh(mcg_solver *s);
 
+/* Initialise at rest with a field of n tanh spheres (an extension, the
+ * interface-rich workload): blue fraction = max over the spheres k of
+ * (1 + tanh((R_k - r_k)/interface_width))/2 at every node (ghosts included),
+ * rho_r = (1 - frac) rhor0, rho_b = frac rhob0, f_i = w_i rho.  Centres and
+ * radii come from mcg_sphere_list(seed, n, NX, NY, NZ, rmin, rmax). */
+int mcg_init_spheres(mcg_solver *s, int n, uint32_t seed, double rmin, double rmax);
+
+/* The spheres of mcg_init_spheres (host only): out[4k..4k+3] = x, y, z, R of
+ * sphere k, uniform in [1, N-2] per axis and in [rmin, rmax], from SplitMix64:
+ * key_k = sm(sm(seed) ^ k), u_j = (sm(key_k + j) >> 40) / 2^24 for j = 1..4. */
+int mcg_sphere_list(uint32_t seed, int n, int NX, int NY, int NZ, double rmin, double rmax, double *out);
+
 /* ---- the hot path ---------------------------------------------------- */
 
 /* Advance nsteps time steps.  One step is the body of the reference loop

@@ -8,7 +8,8 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint8, c_uint64, c_void_p
+from ctypes import (POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint8, c_uint32, c_uint64,
+                    c_void_p)
 
 LIB_NAME = "libmcg_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
@@ -119,6 +120,8 @@ SIGNATURES = [
     ("mcg_init_jet", c_int, [_PSOLVER]),
     ("mcg_init_bubble", c_int, [_PSOLVER]),
     ("mcg_init_bubble_tanh", c_int, [_PSOLVER]),
+    ("mcg_init_spheres", c_int, [_PSOLVER, c_int, c_uint32, c_double, c_double]),
+    ("mcg_sphere_list", c_int, [c_uint32, c_int, c_int, c_int, c_int, c_double, c_double, _D]),
     ("mcg_step", c_int, [_PSOLVER, c_int]),
     ("mcg_synchronize", c_int, [_PSOLVER]),
     ("mcg_get_step", c_int, [_PSOLVER, POINTER(c_int64)]),

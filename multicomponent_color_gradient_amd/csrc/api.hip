@@ -40,7 +40,7 @@ int fail(int code, const std::string &msg) {
         if (e_ != hipSuccess) return fail(MCG_E_HIP, std::string("launch ") + name + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kInitJet = 0, kInitBubble = 1, kInitBubbleTanh = 2;
+constexpr int kInitJet = 0, kInitBubble = 1, kInitBubbleTanh = 2, kInitSpheres = 3;
 
 // launch KERNEL<storage> for the run-time storage mode
 #define MCG_DISPATCH_ST(st, KERNEL, ...)                                                 \
@@ -297,8 +297,8 @@ int do_init(mcg_solver *s, int kind) {
     s->geo.g[2] = float(c.bubble_z0);
     s->geo.g[3] = float(c.bubble_radius);
     s->geo.g[4] = float(c.interface_width);
-    if (kind == kInitBubbleTanh && !(c.interface_width > 0))
-        return fail(MCG_E_INVALID, "interface_width must be > 0 for the tanh bubble");
+    if ((kind == kInitBubbleTanh || kind == kInitSpheres) && !(c.interface_width > 0))
+        return fail(MCG_E_INVALID, "interface_width must be > 0 for the tanh bubble and spheres");
     s->cur = 0;
     for (int b = 0; b < 2; ++b) {
         HIP_TRY(hipMemsetAsync(s->bin[b], 0, s->bnd_bytes(), s->stream));
@@ -569,6 +569,43 @@ int mcg_set_kernel(mcg_solver *s, int kernel) {
 int mcg_init_jet(mcg_solver *s) { return do_init(s, kInitJet); }
 int mcg_init_bubble(mcg_solver *s) { return do_init(s, kInitBubble); }
 int mcg_init_bubble_tanh(mcg_solver *s) { return do_init(s, kInitBubbleTanh); }
+
+int mcg_sphere_list(uint32_t seed, int n, int NX, int NY, int NZ, double rmin, double rmax, double *out) {
+    if (!out || n < 1 || NX < 3 || NY < 3 || NZ < 3 || !(rmin > 0) || rmax < rmin)
+        return fail(MCG_E_INVALID, "sphere list: need n >= 1, a grid of at least 3^3 and 0 < rmin <= rmax");
+    const uint64_t base = splitmix64_host(uint64_t(seed));
+    auto u = [](uint64_t key, int j) { return double(splitmix64_host(key + uint64_t(j)) >> 40) / 16777216.0; };
+    for (int k = 0; k < n; ++k) {
+        const uint64_t key = splitmix64_host(base ^ uint64_t(k));
+        out[4 * k + 0] = 1.0 + u(key, 1) * (NX - 3);
+        out[4 * k + 1] = 1.0 + u(key, 2) * (NY - 3);
+        out[4 * k + 2] = 1.0 + u(key, 3) * (NZ - 3);
+        out[4 * k + 3] = rmin + u(key, 4) * (rmax - rmin);
+    }
+    return MCG_OK;
+}
+
+int mcg_init_spheres(mcg_solver *s, int n, uint32_t seed, double rmin, double rmax) {
+    if (int rc = check_solver(s)) return rc;
+    if (n > 65536) return fail(MCG_E_INVALID, "at most 65536 spheres");
+    std::vector<double> d(4 * size_t(std::max(n, 1)));
+    if (int rc = mcg_sphere_list(seed, n, s->c.NX, s->c.NY, s->c.NZ, rmin, rmax, d.data())) return rc;
+    std::vector<float> h(d.begin(), d.end());
+    float *dev = nullptr;
+    HIP_TRY(hipMalloc((void **)&dev, h.size() * sizeof(float)));
+    int rc = MCG_OK;
+    if (hipMemcpy(dev, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(MCG_E_HIP, "sphere list upload failed");
+    } else {
+        s->geo.sph = dev;
+        s->geo.nsph = n;
+        rc = do_init(s, kInitSpheres);
+    }
+    s->geo.sph = nullptr;
+    s->geo.nsph = 0;
+    (void)hipFree(dev);
+    return rc;
+}
 
 int mcg_step(mcg_solver *s, int nsteps) {
     if (int rc = check_ready(s)) return rc;

@@ -15,9 +15,15 @@ namespace mcg {
 
 // ---------------------------------------------------------------- init
 
-// initial colour densities, jet/jetIn.cuh:13-62, bubble/bubbleIn.cuh:10-23
-// and the tanh bubble extension
-__device__ __forceinline__ void init_rho(const DevParams &P, int kind, const float *geo, int x, int y, int z,
+struct InitGeo {
+    float g[5];          // bubble_x0, bubble_y0, bubble_z0, bubble_radius, interface_width
+    const float *sph;    // kind 3: nsph spheres (x, y, z, R)
+    int nsph;
+};
+
+// initial colour densities, jet/jetIn.cuh:13-62, bubble/bubbleIn.cuh:10-23,
+// the tanh bubble and the field of tanh spheres (extensions)
+__device__ __forceinline__ void init_rho(const DevParams &P, int kind, const InitGeo &geo, int x, int y, int z,
                                          float &rr, float &rb) {
     if (kind == 0) {
         rr = P.rhor0;
@@ -29,23 +35,31 @@ __device__ __forceinline__ void init_rho(const DevParams &P, int kind, const flo
         }
         return;
     }
-    // geo = {bubble_x0, bubble_y0, bubble_z0, bubble_radius, interface_width}
-    const float dx = float(x) - geo[0], dy = float(y) - geo[1], dz = float(z) - geo[2];
+    if (kind == 3) {
+        // blue fraction = max over the spheres of (1 + tanh((R_k - r_k)/w))/2
+        float frac = 0.0f;
+        for (int k = 0; k < geo.nsph; ++k) {
+            const float dx = float(x) - geo.sph[4 * k], dy = float(y) - geo.sph[4 * k + 1], dz = float(z) - geo.sph[4 * k + 2];
+            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            frac = fmaxf(frac, (1.0f + tanhf((geo.sph[4 * k + 3] - r) / geo.g[4])) / 2.0f);
+        }
+        rr = (1.0f - frac) * P.rhor0;
+        rb = frac * P.rhob0;
+        return;
+    }
+    const float *g = geo.g;
+    const float dx = float(x) - g[0], dy = float(y) - g[1], dz = float(z) - g[2];
     if (kind == 1) {
-        const float inb = ((dx * dx) + (dy * dy) + (dz * dz)) <= (geo[3] * geo[3]) ? 1.0f : 0.0f;
+        const float inb = ((dx * dx) + (dy * dy) + (dz * dz)) <= (g[3] * g[3]) ? 1.0f : 0.0f;
         rr = (1.0f - inb) * P.rhor0;
         rb = inb * P.rhob0;
     } else {
         const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float frac = (1.0f + tanhf((geo[3] - r) / geo[4])) / 2.0f;
+        const float frac = (1.0f + tanhf((g[3] - r) / g[4])) / 2.0f;
         rr = (1.0f - frac) * P.rhor0;
         rb = frac * P.rhob0;
     }
 }
-
-struct InitGeo {
-    float g[5];
-};
 
 // Populations f_i = w_i rho_k at every interior node, stored at the pull
 // source of each slot (initial state of jetDensity / bubble, kernels.cu:13-42).
@@ -53,7 +67,7 @@ template <int ST>
 __global__ __launch_bounds__(256) void k_init_pull(DevParams P, int kind, InitGeo geo, void *f, float *bin, float *bout) {
     MCG_INTERIOR_XYZ(P, x, y, z)
     float rr, rb;
-    init_rho(P, kind, geo.g, x, y, z, rr, rb);
+    init_rho(P, kind, geo, x, y, z, rr, rb);
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
         scatter<ST, I>(P, f, bin, bout, x, y, z, w(I) * rr, w(I) * rb);
@@ -68,7 +82,7 @@ __global__ __launch_bounds__(256) void k_init_ghost(DevParams P, int kind, InitG
     const int z = int(blockIdx.z);
     if (x >= P.NX || y >= P.NY) return;
     float rr, rb;
-    init_rho(P, kind, geo.g, x, y, z, rr, rb);
+    init_rho(P, kind, geo, x, y, z, rr, rb);
     const size_t g = gidx(P, x, y, z);
     grr[g] = rr;
     grb[g] = rb;

@@ -74,6 +74,10 @@ class Solver:
     def init_bubble_tanh(self) -> None:
         check(self.lib.mcg_init_bubble_tanh(self._h))
 
+    def init_spheres(self, n: int = 256, seed: int = 7, rmin: float = 6.0, rmax: float = 14.0) -> None:
+        """At rest with a field of n seeded tanh spheres (mcg_init_spheres): the interface-rich workload."""
+        check(self.lib.mcg_init_spheres(self._h, int(n), int(seed), float(rmin), float(rmax)))
+
     # ---- hot path
     def step(self, n: int = 1) -> None:
         check(self.lib.mcg_step(self._h, int(n)))
@@ -243,3 +247,11 @@ def jetIn(solver: Solver) -> None:
 def bubbleIn(solver: Solver) -> None:
     """Sharp bubble initial state: bubble (kernels.cu:13-27)."""
     solver.init_bubble()
+
+
+def sphere_list(seed: int, n: int, NX: int, NY: int, NZ: int, rmin: float, rmax: float) -> np.ndarray:
+    """Centres and radii (n, 4) of the spheres mcg_init_spheres places (host computation)."""
+    out = np.empty((n, 4), np.float64)
+    check(load_library().mcg_sphere_list(int(seed), int(n), int(NX), int(NY), int(NZ), float(rmin), float(rmax),
+                                         as_ptr(out, ctypes.c_double)))
+    return out
