@@ -142,13 +142,6 @@ constexpr float invw(int i) { return 1.0f / w(i); }
 __device__ __forceinline__ uint32_t f32_bits(float v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ float bits_f32(uint32_t v) { return __builtin_bit_cast(float, v); }
 
-// float -> bf16 bits, round to nearest even (NaN kept quiet)
-__device__ __forceinline__ uint32_t bf16_bits(float v) {
-    const uint32_t u = f32_bits(v);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
-    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-}
-
 template <int ST>
 struct Store;
 
@@ -198,13 +191,6 @@ struct Store16 {
     __device__ __forceinline__ static uint32_t *at(void *f, uint32_t idx) {
         return reinterpret_cast<uint32_t *>(static_cast<char *>(f) + (idx << 2));
     }
-    __device__ __forceinline__ static uint32_t half_bits(float v) {
-        if constexpr (ST == MCG_STORE_FP16) {
-            return uint32_t(__builtin_bit_cast(uint16_t, static_cast<_Float16>(v)));
-        } else {
-            return bf16_bits(v);
-        }
-    }
     __device__ __forceinline__ static float half_value(uint32_t h) {
         if constexpr (ST == MCG_STORE_FP16) {
             return float(__builtin_bit_cast(_Float16, uint16_t(h)));
@@ -226,13 +212,28 @@ struct Store16 {
             fr = t - m;
         }
     }
+    // The word: low half a (round to nearest even), high half the minority
+    // magnitude with its sign bit set when red is the minority.  Built as the
+    // float pair (a, +-|m|) and converted in one packed instruction
+    // (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32): rounding is symmetric, so the
+    // sign of -|m| lands exactly on the flag bit.
+    __device__ __forceinline__ static uint32_t pack(float lo, float hi) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        if constexpr (ST == MCG_STORE_FP16) {
+            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+            return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{lo, hi}, h2));
+        } else {
+            typedef __bf16 h2 __attribute__((ext_vector_type(2)));
+            return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{lo, hi}, h2));
+        }
+    }
     template <int I>
     __device__ __forceinline__ static uint32_t encode(float fr, float fb) {
         const float t = fr + fb;
         const float a = t * invw(I) - 1.0f;
         const bool red_major = fr >= fb;
-        const float m = fabsf(red_major ? fb : fr) * invw(I);
-        return half_bits(a) | ((half_bits(m) | (red_major ? 0u : 0x8000u)) << 16);
+        const float m = red_major ? fabsf(fb) * invw(I) : -(fabsf(fr) * invw(I));
+        return pack(a, m);
     }
     template <int I>
     __device__ __forceinline__ static void load(const void *f, uint32_t stride, uint32_t off, float &fr, float &fb) {
@@ -254,8 +255,7 @@ struct Store16 {
     // word from the offset a = t/w - 1 and the colours already divided by w
     __device__ __forceinline__ static uint32_t encode_w(float a, float rw, float bw) {
         const bool red_major = rw >= bw;
-        const float m = fabsf(red_major ? bw : rw);
-        return half_bits(a) | ((half_bits(m) | (red_major ? 0u : 0x8000u)) << 16);
+        return pack(a, red_major ? fabsf(bw) : -fabsf(rw));
     }
     template <int I>
     __device__ __forceinline__ static void store_w(void *f, uint32_t stride, uint32_t off, float a, float rw, float bw) {
