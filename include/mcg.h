@@ -83,7 +83,11 @@ typedef struct mcg_case {
     int32_t viscosity_mode;      /* MCG_VISC_*                                   */
     int32_t inlet_profile;       /* MCG_INLET_*                                  */
     int32_t perturbation_seed;   /* seed of the inlet perturbation (MCG_INLET_TANH) */
-    int32_t reserved_i[3];
+    int32_t wall_xz;             /* 1: no-slip walls (half-way bounce-back) on the x and z
+                                    ghost layers instead of periodic x/z (y_mode periodic
+                                    only); the colour gradient reads the frozen ghost
+                                    densities there (an extension)                  */
+    int32_t reserved_i[2];
 
     double jet_radius, jet_x0, jet_z0, jet_velocity;       /* constants.cuh:32-35 */
     double bubble_radius, bubble_x0, bubble_y0, bubble_z0; /* constants.cuh:38-41 */
@@ -97,7 +101,12 @@ typedef struct mcg_case {
     double sponge_K;             /* utilities/mathUtilities.cuh:50 (reference 0)  */
     double interface_width;      /* tanh width for mcg_init_bubble_tanh / inlet   */
     double perturbation_amp;     /* inlet transverse perturbation, fraction of U  */
-    double reserved_d[4];
+    double gravity;              /* body force F = (0, -gravity rho_r, 0) on the red
+                                    (heavy) component (an extension): u = (j + F/2)/rho
+                                    and a source w_i [(1 - omega/2)(9 (c_i.u) c_i - 3 u)
+                                    + 1.5 c_i].F in the collision, so that the
+                                    post-collision momentum is j + F               */
+    double reserved_d[3];
 } mcg_case;
 
 typedef struct mcg_solver mcg_solver;

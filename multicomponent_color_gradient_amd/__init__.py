@@ -8,7 +8,7 @@ Python binding of that ABI plus the reference's case API and output formats.
 """
 from ._abi import McgCase, McgError, load_library, library_path  # noqa: F401
 from .case import (derived, interior_nodes, jet_case, periodic_bubble_case, perturbed_jet_case,  # noqa: F401
-                   reference_jet)
+                   reference_jet, rising_bubble_case)
 from .solver import Solver, bubbleIn, jetIn  # noqa: F401
 
 __version__ = "0.1.0"

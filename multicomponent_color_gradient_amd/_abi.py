@@ -65,7 +65,8 @@ class McgCase(ctypes.Structure):
         ("viscosity_mode", c_int32),
         ("inlet_profile", c_int32),
         ("perturbation_seed", c_int32),
-        ("reserved_i", c_int32 * 3),
+        ("wall_xz", c_int32),
+        ("reserved_i", c_int32 * 2),
         ("jet_radius", c_double),
         ("jet_x0", c_double),
         ("jet_z0", c_double),
@@ -86,7 +87,8 @@ class McgCase(ctypes.Structure):
         ("sponge_K", c_double),
         ("interface_width", c_double),
         ("perturbation_amp", c_double),
-        ("reserved_d", c_double * 4),
+        ("gravity", c_double),
+        ("reserved_d", c_double * 3),
     ]
 
     def copy(self) -> "McgCase":

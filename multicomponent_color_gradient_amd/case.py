@@ -112,6 +112,34 @@ def periodic_bubble_case(N: int, radius: float, tau: float = 0.8, A: float = 1e-
     return c
 
 
+def rising_bubble_case(NX: int = 192, NY: int = 384, NZ: int = 192, radius: float = 16.0, y0: float = 64.0,
+                       gravity: float = 1e-5, taur: float = 0.55, taub: float = 1.0, A: float = 1e-3,
+                       beta: float = 0.7, width: float = 2.0, **overrides) -> McgCase:
+    """Buoyant rising bubble (an extension): no-slip walls in x and z, periodic y,
+    a blue tanh bubble of radius R centred at ((NX-1)/2, y0, (NZ-1)/2) in red, the
+    body force F_y = -gravity rho_r on the red (heavy) component, per-node
+    viscosity between taur (red) and taub (blue), A given directly.  Start it
+    with init_bubble_tanh."""
+    c = reference_jet()
+    c.NX, c.NY, c.NZ = NX, NY, NZ
+    c.y_mode = MCG_Y_PERIODIC
+    c.wall_xz = 1
+    c.wrap_gradient = 1
+    c.viscosity_mode = MCG_VISC_INTERFACE
+    c.taur, c.taub = float(taur), float(taub)
+    c.A_override = float(A)
+    c.beta_recolor = float(beta)
+    c.gravity = float(gravity)
+    c.interface_width = float(width)
+    c.bubble_radius = float(radius)
+    c.bubble_x0, c.bubble_y0, c.bubble_z0 = (NX - 1) / 2.0, float(y0), (NZ - 1) / 2.0
+    c.jet_x0, c.jet_z0 = (NX - 1) / 2.0, (NZ - 1) / 2.0
+    c.NSTATS_START_STEP = 1 << 30
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
 def derived(c: McgCase) -> dict:
     """Derived constants of a case (constants.cuh:59-76, forces.cuh:87-105)."""
     nu = (c.taub - 0.5) * CS2

@@ -248,6 +248,9 @@ void fill_params(const mcg_case &c, int storage, DevParams &P) {
     P.inlet_w = float(c.interface_width);
     P.pert_amp = float(c.perturbation_amp);
     P.pert_seed = uint32_t(c.perturbation_seed);
+    P.wall = c.wall_xz;
+    P.ghalf = float(0.5 * c.gravity);
+    P.grav = float(c.gravity);
 }
 
 int check_solver(const mcg_solver *s) {
@@ -472,6 +475,9 @@ int mcg_case_validate(const mcg_case *c) {
     if (!(c->taub > 0.5) || !(c->taur > 0.5)) return fail(MCG_E_INVALID, "relaxation times must exceed 0.5");
     if (c->sponge_cells < 2 && c->sponge_K != 0.0) return fail(MCG_E_INVALID, "sponge_cells must be >= 2");
     if (c->rhor0 <= 0 || c->rhob0 <= 0) return fail(MCG_E_INVALID, "reference densities must be > 0");
+    if (c->wall_xz != 0 && c->wall_xz != 1) return fail(MCG_E_INVALID, "wall_xz must be 0 or 1");
+    if (c->wall_xz && c->y_mode != MCG_Y_PERIODIC) return fail(MCG_E_UNSUPPORTED, "x/z walls need y_mode periodic");
+    if (!std::isfinite(c->gravity)) return fail(MCG_E_INVALID, "gravity must be finite");
     return MCG_OK;
 }
 
@@ -909,6 +915,7 @@ static int adj_supported(const mcg_case &c, int storage) {
         return fail(MCG_E_UNSUPPORTED, "the adjoint covers jet cases and the periodic domain (y_mode periodic, wrap_gradient 1)");
     }
     if (c.viscosity_mode != MCG_VISC_SINGLE) return fail(MCG_E_UNSUPPORTED, "the adjoint covers the single relaxation rate");
+    if (c.wall_xz || c.gravity != 0.0) return fail(MCG_E_UNSUPPORTED, "the adjoint does not cover walls or body forces");
     if (storage != MCG_STORE_FP32) return fail(MCG_E_UNSUPPORTED, "the adjoint runs on FP32 populations");
     return MCG_OK;
 }
@@ -1293,6 +1300,7 @@ int mcg_cpu_adjoint_jet(const mcg_case *c, int nsteps, const double *fr0, const 
     if (!c || !fr0 || !fb0 || !fields0) return fail(MCG_E_INVALID, "null pointer");
     if (int rc = mcg_case_validate(c)) return rc;
     if (c->y_mode != MCG_Y_JET || c->wrap_gradient) return fail(MCG_E_UNSUPPORTED, "the jet adjoint covers y_mode jet with the reference gradient");
+    if (c->gravity != 0.0) return fail(MCG_E_UNSUPPORTED, "the jet adjoint does not cover body forces");
     if (c->viscosity_mode != MCG_VISC_SINGLE || c->sponge_K != 0.0)
         return fail(MCG_E_UNSUPPORTED, "the CPU adjoint assumes a uniform relaxation rate");
     if (nsteps < 0) return fail(MCG_E_INVALID, "nsteps < 0");

@@ -58,6 +58,8 @@ struct DevParams {
     float inlet_w;       // tanh width of the MCG_INLET_TANH profile
     float pert_amp;      // transverse perturbation amplitude, fraction of jet_U
     uint32_t pert_seed;  // seed of the perturbation
+    int wall;            // 1: bounce-back walls on the x and z ghost layers
+    float grav, ghalf;   // body force F_y = -grav rho_r on red; ghalf = grav/2
 };
 
 struct DevBufs {
@@ -269,7 +271,8 @@ struct Store<MCG_STORE_BF16> : Store16<MCG_STORE_BF16> {};
 
 // Source of the population that streams into interior node (x,y,z) along
 // direction I: kind 0 = interior node (off = its plane offset), 1 / 2 = the
-// inlet / outlet source buffer (off = element of colour r, colour b at +NX*NZ).
+// inlet / outlet source buffer (off = element of colour r, colour b at +NX*NZ),
+// 3 = bounce-back at an x/z wall (off = the node itself, direction opp(I)).
 struct Src {
     int kind;
     size_t off;
@@ -280,6 +283,9 @@ __device__ __forceinline__ Src source_of(const DevParams &P, int x, int y, int z
     int xs = x - kCX[I];
     int ys = y - kCY[I];
     int zs = z - kCZ[I];
+    // half-way bounce-back: a population arriving from a wall ghost is the
+    // node's own post-collision population of the opposite direction
+    if (P.wall && (xs < 1 || xs > P.NX - 2 || zs < 1 || zs > P.NZ - 2)) return Src{3, size_t(pidx(P, x, y, z))};
     if constexpr (kCY[I] != 0) {
         if (P.y_mode == MCG_Y_JET) {
             const bool ghost = kCY[I] > 0 ? (ys == 0) : (ys == P.NY - 1);
@@ -304,6 +310,8 @@ __device__ __forceinline__ void gather(const DevParams &P, const void *__restric
     const Src s = source_of<I>(P, x, y, z);
     if (s.kind == 0) {
         Store<ST>::template load<I>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb);
+    } else if (s.kind == 3) {
+        Store<ST>::template load<opp(I)>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb);
     } else {
         const float *b = s.kind == 1 ? bin : bout;
         fr = b[s.off];
@@ -317,6 +325,8 @@ __device__ __forceinline__ void scatter(const DevParams &P, void *f, float *bin,
     const Src s = source_of<I>(P, x, y, z);
     if (s.kind == 0) {
         Store<ST>::template store<I>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb);
+    } else if (s.kind == 3) {
+        Store<ST>::template store<opp(I)>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb);
     } else {
         float *b = s.kind == 1 ? bin : bout;
         b[s.off] = fr;
@@ -336,7 +346,7 @@ __device__ __forceinline__ Moments gather_moments(const DevParams &P, const void
         gather<ST, I>(P, f, bin, bout, x, y, z, a, b);
         acc.add<I>(a, b);
     });
-    return acc.finish();
+    return acc.finish(P.ghalf);
 }
 
 __device__ __forceinline__ bool interior(const DevParams &P, int x, int y, int z) {
@@ -359,9 +369,11 @@ __device__ __forceinline__ float ghost_psi(const DevParams &P, const DevBufs &B,
 // neighbour coordinates for the colour gradient (wrapped when wrap_gradient)
 __device__ __forceinline__ void grad_neighbor(const DevParams &P, int x, int y, int z, int &xn, int &yn, int &zn) {
     xn = x; yn = y; zn = z;
-    if (P.wrap_gradient) {
-        xn = wrapN(x, P.NX);
-        zn = wrapN(z, P.NZ);
+    if (P.wrap_gradient) {  // along the periodic axes
+        if (!P.wall) {
+            xn = wrapN(x, P.NX);
+            zn = wrapN(z, P.NZ);
+        }
         if (P.y_mode == MCG_Y_PERIODIC) yn = wrapN(y, P.NY);
     }
 }
@@ -381,6 +393,16 @@ __device__ __forceinline__ NodeColl node_setup(const DevParams &P, const Moments
     n.uz = m.uz;
     const float usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
     n.e0 = rT * (1.0f - usq * inv_2cs2);
+    {
+        // body force on red, F_y = -grav rho_r: even source (1 - omega/2)(9 (c.u) c_y - 3 u_y) F_y,
+        // odd 1.5 c_y F_y (the regularised collision drops the first-order non-equilibrium
+        // moment -F/2, whose (1 - omega) share the odd source restores: sum c f* = j + F)
+        const float Fy = -P.grav * m.rr;
+        const float gy = (0.5f + 0.5f * oms) * Fy;
+        n.g9 = 9.0f * gy;
+        n.g3 = 1.5f * Fy;
+        n.e0 = fma_(-3.0f * gy, m.uy, n.e0);
+    }
     n.e1 = rT * inv_2cs4;
     n.o1 = rT * (inv_cs2 - usq * inv_2cs4);
     n.o3 = rT * inv_6cs6;

@@ -50,16 +50,22 @@ struct FastSrc {
     uint32_t wxp, wxm, wyp, wym, wzp, wzm;  // modulo-2^32 offset corrections
     int o;
     bool yin, yout;
+    bool bxp, bxm, bzp, bzm;  // x/z walls: sources across x=0, x=NX-1, z=0, z=NZ-1 bounce back
 };
 
 __device__ __forceinline__ FastSrc make_src(const DevParams &P, int x, int y, int z) {
     FastSrc s;
     s.p = pidx(P, x, y, z);
     const uint32_t dy = P.rowstride, dz = P.rowstride * uint32_t(P.IY);
-    s.wxp = (x == 1) ? uint32_t(P.NX - 2) : 0u;
-    s.wxm = (x == P.NX - 2) ? 0u - uint32_t(P.NX - 2) : 0u;
-    s.wzp = (z == 1) ? uint32_t(P.NZ - 2) * dz : 0u;
-    s.wzm = (z == P.NZ - 2) ? 0u - uint32_t(P.NZ - 2) * dz : 0u;
+    const bool wall = P.wall != 0;
+    s.wxp = (!wall && x == 1) ? uint32_t(P.NX - 2) : 0u;
+    s.wxm = (!wall && x == P.NX - 2) ? 0u - uint32_t(P.NX - 2) : 0u;
+    s.wzp = (!wall && z == 1) ? uint32_t(P.NZ - 2) * dz : 0u;
+    s.wzm = (!wall && z == P.NZ - 2) ? 0u - uint32_t(P.NZ - 2) * dz : 0u;
+    s.bxp = wall && x == 1;
+    s.bxm = wall && x == P.NX - 2;
+    s.bzp = wall && z == 1;
+    s.bzm = wall && z == P.NZ - 2;
     const bool jet = P.y_mode == MCG_Y_JET;
     s.wyp = (!jet && y == 1) ? uint32_t(P.NY - 2) * dy : 0u;
     s.wym = (!jet && y == P.NY - 2) ? 0u - uint32_t(P.NY - 2) * dy : 0u;
@@ -92,6 +98,17 @@ __device__ __forceinline__ void fast_load(const DevParams &P, const void *__rest
             return;
         }
     }
+    if constexpr (cx != 0 || cz != 0) {
+        bool bb = false;
+        if constexpr (cx > 0) bb = bb || s.bxp;
+        if constexpr (cx < 0) bb = bb || s.bxm;
+        if constexpr (cz > 0) bb = bb || s.bzp;
+        if constexpr (cz < 0) bb = bb || s.bzm;
+        if (bb) {  // half-way bounce-back
+            Store<ST>::template load<opp(I)>(f, uint32_t(P.PX), s.p, fr, fb);
+            return;
+        }
+    }
     // unsigned arithmetic: intermediate values may wrap, the final offset is in range
     uint32_t off = s.p - uint32_t(cx) - uint32_t(cy) * P.rowstride - uint32_t(cz) * (P.rowstride * uint32_t(P.IY));
     if constexpr (cx > 0) off += s.wxp;
@@ -114,7 +131,7 @@ __device__ __forceinline__ Moments fast_moments(const DevParams &P0, const DevBu
         fast_load<ST, I>(P, B.f_src, B.in_src, B.out_src, s, a, b);
         acc.add<I>(a, b);
     });
-    return acc.finish();
+    return acc.finish(P.ghalf);
 }
 
 // psi of any grid position as the colour gradient sees it (0 off the grid,
@@ -166,7 +183,7 @@ __device__ __forceinline__ void raw_issue(const DevParams &P0, const DevBufs &B,
 }
 
 template <int ST>
-__device__ __forceinline__ Moments raw_reduce(const RawGather<ST> &r) {
+__device__ __forceinline__ Moments raw_reduce(const RawGather<ST> &r, float ghalf) {
     MomentAcc acc;
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
@@ -174,7 +191,7 @@ __device__ __forceinline__ Moments raw_reduce(const RawGather<ST> &r) {
         Store<ST>::template decode_raw<I>(r.w + I * Store<ST>::kWords, a, b);
         acc.add<I>(a, b);
     });
-    return acc.finish();
+    return acc.finish(ghalf);
 }
 
 struct FusedGrid {
@@ -285,7 +302,8 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     bool on_grid = gx >= 0 && gx < P.NX && gy >= 0 && gy < P.NY;
     if (on_grid) grad_neighbor(P, gx, gy, 1, gxn, gyn, gzn_unused);
     const bool split_xy = active && on_grid && gxn >= 1 && gxn <= P.NX - 2 && gyn >= 1 && gyn <= P.NY - 2 &&
-                          !(P.y_mode == MCG_Y_JET && (gyn == 1 || gyn == P.NY - 2));
+                          !(P.y_mode == MCG_Y_JET && (gyn == 1 || gyn == P.NY - 2)) &&
+                          !(P.wall && (gxn == 1 || gxn == P.NX - 2));
 
     Moments cur{}, nx1{}, nx2{};
     RawGather<ST> raw;
@@ -299,8 +317,8 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         const int zl = z + 2;
         const bool lead = zl <= ze + 1;
         int zln = zl;
-        if (P.wrap_gradient) zln = wrapN(zl, P.NZ);
-        const bool split = lead && split_xy && zln >= 1 && zln <= P.NZ - 2;
+        if (P.wrap_gradient && !P.wall) zln = wrapN(zl, P.NZ);
+        const bool split = lead && split_xy && zln >= 1 && zln <= P.NZ - 2 && !(P.wall && (zln == 1 || zln == P.NZ - 2));
         if (split) raw_issue<ST>(P, B, gxn, gyn, zln, raw);
         if (node && z >= zb) {
             const Moments &m = cur;
@@ -344,7 +362,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         if (lead && active) {
             float ph;
             if (split) {
-                nx2 = raw_reduce<ST>(raw);
+                nx2 = raw_reduce<ST>(raw, P.ghalf);
                 ph = psi(nx2.rr, nx2.rb);
             } else {
                 bool has_m;

@@ -40,6 +40,9 @@ constexpr float inv_6cs6 = 1.0f / (6.0f * cs6);
 constexpr float inv_2cs6 = 1.0f / (2.0f * cs6);
 
 constexpr int csq(int i) { return kCX[i] * kCX[i] + kCY[i] * kCY[i] + kCZ[i] * kCZ[i]; }
+// opposite direction: the directions 2k-1 and 2k (k = 1..13) are opposite
+constexpr int opp(int i) { return i == 0 ? 0 : (i % 2 ? i + 1 : i - 1); }
+static_assert(kCX[opp(25)] == -kCX[25] && kCY[opp(13)] == -kCY[13] && kCZ[opp(22)] == -kCZ[22], "opposites");
 
 constexpr float w(int i) {
     return csq(i) == 0 ? float(8.0 / 27.0) : csq(i) == 1 ? float(2.0 / 27.0) : csq(i) == 2 ? float(1.0 / 54.0) : float(1.0 / 216.0);

@@ -97,14 +97,16 @@ struct MomentAcc {
         if constexpr (Hxz(I) > 0.0f) sxz += g; else if constexpr (Hxz(I) < 0.0f) sxz -= g;
     }
 
-    __device__ __forceinline__ Moments finish() const {
+    // ghalf = gravity/2: with the body force F_y = -gravity rho_r the velocity
+    // is (j + F/2)/rho (Guo); ghalf = 0 leaves every value bit-identical
+    __device__ __forceinline__ Moments finish(float ghalf) const {
         Moments m;
         m.rr = sr;
         m.rb = sb;
         const float rho = sr + sb;
         const float inv = 1.0f / rho;
         m.ux = jx * inv;
-        m.uy = jy * inv;
+        m.uy = (jy - ghalf * sr) * inv;
         m.uz = jz * inv;
         m.pxx = sxx - rho * m.ux * m.ux;
         m.pxy = sxy - rho * m.ux * m.uy;
@@ -165,6 +167,7 @@ struct NodeColl {
     float Fx, Fy, Fz;
     float s2, sB;          // Omega2/w = s2 (c.F)^2 - sB
     float rc;              // D = rc (c.F)/|c|
+    float g9, g3;          // body force: E += g9 c_y (c.u), O += g3 c_y (g9 = 9 (1 - omega/2) F_y, g3 = 1.5 F_y)
 };
 
 __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -206,6 +209,15 @@ __device__ __forceinline__ void pair_terms(const NodeColl &n, float &E, float &O
     E = fma_(n.s2, d * d, E) - n.sB;
     O = cu * fma_(n.o3, cu2, n.o1);
     O = fma_(n.q3, a3, O);
+    // body force F = (0, F_y, 0): even part (1 - omega/2)(9 (c.u)(c.F) - 3 u.F) (its
+    // u-only term sits in e0), odd part 1.5 c.F
+    if constexpr (kCY[I] > 0) {
+        E = fma_(n.g9, cu, E);
+        O = O + n.g3;
+    } else if constexpr (kCY[I] < 0) {
+        E = fma_(-n.g9, cu, E);
+        O = O - n.g3;
+    }
     D = n.rc * d;
     if constexpr (csq(I) > 1) D = D * invc(I);
 }

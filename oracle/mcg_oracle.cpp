@@ -37,6 +37,8 @@ constexpr int Q = 27;
 
 // D3Q27 velocities in the reference's order: 0 rest, 1-6 faces, 7-18 edges,
 // 19-26 corners, opposite directions adjacent (stencil_ct.cuh:9-16).
+// opposite direction (the directions 2k-1 and 2k are opposite)
+constexpr int opposite(int i) { return i == 0 ? 0 : (i % 2 ? i + 1 : i - 1); }
 constexpr int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
 constexpr int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
 constexpr int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
@@ -351,7 +353,8 @@ struct Oracle final : OracleBase {
                     rhob[id] = sb;
                     const T rho = sr + sb;
                     const T inv = T(1) / rho;
-                    const T vx = jx * inv, vy = jy * inv, vz = jz * inv;
+                    // Guo forcing (an extension): u = (j + F/2)/rho, F = (0, -gravity rho_r, 0)
+                    const T vx = jx * inv, vy = (jy - T(0.5) * T(c.gravity) * sr) * inv, vz = jz * inv;
                     ux[id] = vx;
                     uy[id] = vy;
                     uz[id] = vz;
@@ -370,9 +373,11 @@ struct Oracle final : OracleBase {
         T sx = 0, sy = 0, sz = 0;
         for (int i = 1; i < Q; ++i) {
             int xn = x + CX[i], yn = y + CY[i], zn = z + CZ[i];
-            if (c.wrap_gradient) {
-                xn = wrapx(xn);
-                zn = wrapz(zn);
+            if (c.wrap_gradient) {  // along the periodic axes
+                if (!c.wall_xz) {
+                    xn = wrapx(xn);
+                    zn = wrapz(zn);
+                }
                 if (c.y_mode == MCG_Y_PERIODIC) yn = wrapy(yn);
             }
             const size_t n = idx(xn, yn, zn);
@@ -409,18 +414,31 @@ struct Oracle final : OracleBase {
                         A = A_node(rr, rb);
                         absF = std::sqrt(Fx * Fx + Fy * Fy + Fz * Fz);
                     }
+                    // body force on red (an extension): Guo source (1 - omega/2) w_i [3 (c_i - u) +
+                    // 9 (c_i.u) c_i].F plus the (1 - omega) share of the first-order non-equilibrium
+                    // moment -F/2 that the regularisation drops: odd part 1.5 w_i c_i.F in total
+                    const T Fb = -T(c.gravity) * rr, sf = T(1) - om / T(2);
                     for (int i = 0; i < Q; ++i) {
                         T g = feq<T>(L, i, rT, vx, vy, vz) + oms * fneqr<T>(L, i, P, vx, vy, vz);
+                        if (c.gravity != 0.0) {
+                            const T cu = T(CX[i]) * vx + T(CY[i]) * vy + T(CZ[i]) * vz;
+                            g = g + L.w[i] * (sf * (T(9) * cu * T(CY[i]) - T(3) * vy) * Fb + T(1.5) * T(CY[i]) * Fb);
+                        }
                         T d = 0;
                         if (!bulk) {
                             g = g + omega2<T>(L, i, Fx, Fy, Fz, absF, A);
                             d = recolor<T>(L, i, rr, rb, Fx, Fy, Fz, absF, beta_);
                         }
-                        const int xn = wrapx(x + CX[i]);
-                        const int zn = wrapz(z + CZ[i]);
-                        int yn = y + CY[i];
-                        if (c.y_mode == MCG_Y_PERIODIC) yn = wrapy(yn);
-                        const size_t s = fidx(idx(xn, yn, zn), i);
+                        size_t s;
+                        if (c.wall_xz && (x + CX[i] < 1 || x + CX[i] > NX - 2 || z + CZ[i] < 1 || z + CZ[i] > NZ - 2)) {
+                            s = fidx(id, opposite(i));  // half-way bounce-back at an x/z wall (an extension)
+                        } else {
+                            const int xn = wrapx(x + CX[i]);
+                            const int zn = wrapz(z + CZ[i]);
+                            int yn = y + CY[i];
+                            if (c.y_mode == MCG_Y_PERIODIC) yn = wrapy(yn);
+                            s = fidx(idx(xn, yn, zn), i);
+                        }
                         if (bulk) {
                             put(fr, cnt_r, s, aR * g);
                             put(fb, cnt_b, s, aB * g);
