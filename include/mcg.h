@@ -299,12 +299,16 @@ int mcg_adjoint_last_ms(const mcg_solver *s, double *ms);
 /* Free the adjoint's checkpoints and buffers. */
 int mcg_adjoint_release(mcg_solver *s);
 
-/* The same forward and adjoint on the CPU in FP64 (host code, no GPU):
- * nsteps steps from (fr0, fb0), final state into (out_fr, out_fb), and, when
- * lam_fr/lam_fb are given, the adjoint at the initial state and dparams. */
-int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *lam_fr,
-                    const double *lam_fb, double *out_fr, double *out_fb, double *out_lam_fr, double *out_lam_fb,
-                    double *dparams);
+/* The same forward and adjoint on the CPU in FP64 (host code, no GPU), for
+ * y_mode periodic (also with x/z walls, per-node viscosity and the body
+ * force): nsteps steps from (fr0, fb0), final state into (out_fr, out_fb),
+ * and, when lam_fr/lam_fb are given, the adjoint at the initial state and
+ * dparams = {dJ/dA, dJ/dbeta, dJ/dgravity}.  fields0 (5 x NX*NY*NZ: rho_r,
+ * rho_b, u of the initial state, ghosts included) gives the frozen ghost psi
+ * of the walls; it may be NULL without walls. */
+int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *fields0,
+                    const double *lam_fr, const double *lam_fb, double *out_fr, double *out_fb, double *out_lam_fr,
+                    double *out_lam_fb, double *dparams);
 
 /* The FP64 CPU forward and adjoint for jet cases (y_mode jet, reference
  * colour gradient, single relaxation rate): the inlet and the convective

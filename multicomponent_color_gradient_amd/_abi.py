@@ -147,7 +147,7 @@ SIGNATURES = [
     ("mcg_adjoint_inlet_gradient", c_int, [_PSOLVER, _D, _D]),
     ("mcg_uy_functional", c_int, [_PSOLVER, _D]),
     ("mcg_set_inlet_offset", c_int, [_PSOLVER, _F]),
-    ("mcg_cpu_adjoint", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
+    ("mcg_cpu_adjoint", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     ("mcg_cpu_adjoint_jet", c_int, [POINTER(McgCase), c_int, _D, _D, _D, _D, _D, _D, _D, c_int, c_int, _D, _D, _D, _D, _D,
                                     _D]),
     ("mcg_time_steps", c_int, [_PSOLVER, c_int, _D]),

@@ -19,16 +19,21 @@ from .solver import Solver
 
 
 def cpu_adjoint(case: McgCase, fr0: np.ndarray, fb0: np.ndarray, nsteps: int, lam_fr: np.ndarray | None = None,
-                lam_fb: np.ndarray | None = None) -> dict:
-    """FP64 forward of nsteps steps and, if lam_* is given, the adjoint: returns
-    {"fr", "fb"} (final state) and {"lam_fr", "lam_fb", "dA", "dbeta"}."""
+                lam_fb: np.ndarray | None = None, fields0: dict | None = None) -> dict:
+    """FP64 forward of nsteps steps (y periodic; also with x/z walls, per-node
+    viscosity and the body force) and, if lam_* is given, the adjoint: returns
+    {"fr", "fb"} (final state) and {"lam_fr", "lam_fb", "dA", "dbeta", "dgravity"}.
+    fields0 (the initial fields dict, ghosts included) is needed with walls."""
     lib = load_library()
     shape = (27, case.NZ, case.NY, case.NX)
     d = ctypes.c_double
     fr0 = np.ascontiguousarray(fr0, np.float64).reshape(shape)
     fb0 = np.ascontiguousarray(fb0, np.float64).reshape(shape)
+    f0 = None
+    if fields0 is not None:
+        f0 = np.ascontiguousarray(np.stack([fields0[k] for k in ("rhor", "rhob", "ux", "uy", "uz")]), np.float64)
     out = {"fr": np.empty(shape), "fb": np.empty(shape)}
-    dp = np.zeros(2)
+    dp = np.zeros(3)
     if lam_fr is not None:
         lam_fr = np.ascontiguousarray(lam_fr, np.float64).reshape(shape)
         lam_fb = np.ascontiguousarray(lam_fb, np.float64).reshape(shape)
@@ -39,10 +44,11 @@ def cpu_adjoint(case: McgCase, fr0: np.ndarray, fb0: np.ndarray, nsteps: int, la
         args = (None, None)
         outs = (None, None, None)
     c = case.copy()
-    check(lib.mcg_cpu_adjoint(ctypes.byref(c), int(nsteps), as_ptr(fr0, d), as_ptr(fb0, d), *args,
-                              as_ptr(out["fr"], d), as_ptr(out["fb"], d), *outs))
+    check(lib.mcg_cpu_adjoint(ctypes.byref(c), int(nsteps), as_ptr(fr0, d), as_ptr(fb0, d),
+                              None if f0 is None else as_ptr(f0, d), *args, as_ptr(out["fr"], d), as_ptr(out["fb"], d),
+                              *outs))
     if lam_fr is not None:
-        out["dA"], out["dbeta"] = float(dp[0]), float(dp[1])
+        out["dA"], out["dbeta"], out["dgravity"] = float(dp[0]), float(dp[1]), float(dp[2])
     return out
 
 

@@ -438,9 +438,27 @@ struct CpuAdjoint {
     int NX, NY, NZ;
     double A, beta, omega;
     size_t N;
+    // extensions: x/z walls (bounce-back, ghost psi frozen at gpsi), per-node
+    // omega and the body force on red
+    bool wall = false;
+    AdjExt<double> ext{};
+    std::vector<double> gpsi;  // psi of every position of the initial state (walls)
 
     size_t id(int x, int y, int z) const { return size_t(x) + size_t(NX) * (size_t(y) + size_t(NY) * size_t(z)); }
     static int wrap(int v, int n) { return v == 0 ? n - 2 : (v == n - 1 ? 1 : v); }
+    bool inner_xz(int x, int z) const { return x >= 1 && x <= NX - 2 && z >= 1 && z <= NZ - 2; }
+    // streaming target of direction i from node (x,y,z); false: bounced back at a wall
+    bool target(int x, int y, int z, int i, size_t &t) const {
+        int tx = x + kCX[i], tz = z + kCZ[i];
+        if (wall) {
+            if (!inner_xz(tx, tz)) return false;
+        } else {
+            tx = wrap(tx, NX);
+            tz = wrap(tz, NZ);
+        }
+        t = id(tx, wrap(y + kCY[i], NY), tz);
+        return true;
+    }
 
     void moments(const double *fr, const double *fb, int x, int y, int z, double m[11]) const {
         const size_t g = id(x, y, z);
@@ -457,6 +475,7 @@ struct CpuAdjoint {
         const double rho = sr + sb;
         m[0] = sr;
         m[1] = sb;
+        j[1] -= 0.5 * ext.grav * sr;  // u = (j + F/2)/rho, F_y = -grav rho_r
         for (int a = 0; a < 3; ++a) m[2 + a] = j[a] / rho;
         const double *u = m + 2;
         m[5 + PXX] = S[PXX] - rho * u[0] * u[0];
@@ -467,10 +486,14 @@ struct CpuAdjoint {
         m[5 + PXZ] = S[PXZ] - rho * u[0] * u[2];
     }
 
-    // moments of every interior node (11 arrays of N) and psi
+    // moments of every interior node (11 arrays of N) and psi of every position
     void all_moments(const double *fr, const double *fb, std::vector<double> &M, std::vector<double> &ps) const {
         M.assign(11 * N, 0.0);
-        ps.assign(N, 0.0);
+        if (wall) {
+            ps = gpsi;
+        } else {
+            ps.assign(N, 0.0);
+        }
         for (int z = 1; z < NZ - 1; ++z)
             for (int y = 1; y < NY - 1; ++y)
                 for (int x = 1; x < NX - 1; ++x) {
@@ -482,18 +505,25 @@ struct CpuAdjoint {
                 }
     }
 
+    // psi read by the colour gradient at neighbour (x,y,z): y wrapped; x/z
+    // wrapped (periodic) or ghost values (walls)
+    double psi_at(const std::vector<double> &ps, int x, int y, int z) const {
+        if (!wall) return ps[id(wrap(x, NX), wrap(y, NY), wrap(z, NZ))];
+        return ps[id(x, wrap(y, NY), z)];
+    }
+
     AdjNode<double> node(const std::vector<double> &M, const std::vector<double> &ps, int x, int y, int z) const {
         const size_t g = id(x, y, z);
         double m[11];
         for (int k = 0; k < 11; ++k) m[k] = M[size_t(k) * N + g];
         double F[3] = {0, 0, 0};
         for (int i = 1; i < Q; ++i) {
-            const double p = ps[id(wrap(x + kCX[i], NX), wrap(y + kCY[i], NY), wrap(z + kCZ[i], NZ))];
+            const double p = psi_at(ps, x + kCX[i], y + kCY[i], z + kCZ[i]);
             F[0] += 3.0 * wT<double>(i) * p * kCX[i];
             F[1] += 3.0 * wT<double>(i) * p * kCY[i];
             F[2] += 3.0 * wT<double>(i) * p * kCZ[i];
         }
-        return adj_setup<double>(m, F[0], F[1], F[2], A, beta, omega);
+        return adj_setup<double>(m, F[0], F[1], F[2], A, beta, omega, ext);
     }
 
     void step(const double *fr, const double *fb, double *gr, double *gb) const {
@@ -508,16 +538,17 @@ struct CpuAdjoint {
                     double r[Q], b[Q];
                     adj_collide<double>(n, r, b);
                     for (int i = 0; i < Q; ++i) {
-                        const size_t t = id(wrap(x + kCX[i], NX), wrap(y + kCY[i], NY), wrap(z + kCZ[i], NZ));
-                        gr[size_t(i) * N + t] = r[i];
-                        gb[size_t(i) * N + t] = b[i];
+                        size_t t;
+                        const size_t s = target(x, y, z, i, t) ? size_t(i) * N + t : size_t(opp(i)) * N + id(x, y, z);
+                        gr[s] = r[i];
+                        gb[s] = b[i];
                     }
                 }
     }
 
     // one adjoint step: lam (adjoint of f^{n+1}) -> out (adjoint of f^n)
     void adjoint(const double *fr, const double *fb, const double *lr, const double *lb, double *outr, double *outb,
-                 double &dA, double &dbeta) const {
+                 double &dA, double &dbeta, double &dgrav) const {
         std::vector<double> M, ps;
         all_moments(fr, fb, M, ps);
         std::vector<double> LM(11 * N, 0.0), LF(3 * N, 0.0);
@@ -527,9 +558,10 @@ struct CpuAdjoint {
                     const AdjNode<double> n = node(M, ps, x, y, z);
                     double mr[Q], mb[Q];
                     for (int i = 0; i < Q; ++i) {
-                        const size_t t = id(wrap(x + kCX[i], NX), wrap(y + kCY[i], NY), wrap(z + kCZ[i], NZ));
-                        mr[i] = lr[size_t(i) * N + t];
-                        mb[i] = lb[size_t(i) * N + t];
+                        size_t t;
+                        const size_t s = target(x, y, z, i, t) ? size_t(i) * N + t : size_t(opp(i)) * N + id(x, y, z);
+                        mr[i] = lr[s];
+                        mb[i] = lb[s];
                     }
                     const AdjOut<double> a = adj_vjp<double>(n, mr, mb);
                     const size_t g = id(x, y, z);
@@ -537,6 +569,7 @@ struct CpuAdjoint {
                     for (int k = 0; k < 3; ++k) LF[size_t(k) * N + g] = a.F[k];
                     dA += a.A;
                     dbeta += a.beta;
+                    dgrav += a.grav;
                 }
         std::fill(outr, outr + size_t(Q) * N, 0.0);
         std::fill(outb, outb + size_t(Q) * N, 0.0);
@@ -546,7 +579,14 @@ struct CpuAdjoint {
                     const size_t g = id(x, y, z);
                     double Lpsi = 0;
                     for (int i = 1; i < Q; ++i) {
-                        const size_t q = id(wrap(x - kCX[i], NX), wrap(y - kCY[i], NY), wrap(z - kCZ[i], NZ));
+                        int sx = x - kCX[i], sz = z - kCZ[i];
+                        if (wall) {
+                            if (!inner_xz(sx, sz)) continue;  // ghost psi is constant
+                        } else {
+                            sx = wrap(sx, NX);
+                            sz = wrap(sz, NZ);
+                        }
+                        const size_t q = id(sx, wrap(y - kCY[i], NY), sz);
                         Lpsi += 3.0 * wT<double>(i) *
                                 (kCX[i] * LF[q] + kCY[i] * LF[N + q] + kCZ[i] * LF[2 * N + q]);
                     }
@@ -559,7 +599,7 @@ struct CpuAdjoint {
                     lam[0] += Lpsi * 2.0 * m[1] / (rho * rho);
                     lam[1] += -Lpsi * 2.0 * m[0] / (rho * rho);
                     double r[Q], b[Q];
-                    adj_moments<double>(m, lam, r, b);
+                    adj_moments<double>(m, lam, r, b, 0.5 * ext.grav, &dgrav);
                     for (int i = 0; i < Q; ++i) {
                         outr[size_t(i) * N + g] = r[i];
                         outb[size_t(i) * N + g] = b[i];

@@ -16,6 +16,11 @@
 // sB = s2 = rc = 0 and F unused at bulk nodes, 4 rr rb/rho^2 <= 1e-4).
 // The adjoint treats the bulk switch and the |F| cut as fixed at the forward
 // state (the map is differentiated away from its clamps).
+// Extensions (AdjExt): the per-node relaxation rate omega = 1/tau(psi),
+// tau = (1+psi)/2 taur + (1-psi)/2 taub (q2, q3 then depend on the node's
+// densities), and the body force F_y = -grav rr on red: e0 gains
+// -3 (1 - omega/2) F_y u_y, E gains 9 (1 - omega/2) F_y c_y (c.u), O gains
+// 1.5 F_y c_y; the moments pass shifts u_y by -grav rr/(2 rho).
 #pragma once
 
 #include "lattice.hpp"
@@ -146,6 +151,13 @@ MCG_HD void fneqr_vjp(int i, const T P[6], T ux, T uy, T uz, T lam, T l_P[6], T 
 }
 
 template <class T>
+struct AdjExt {
+    bool iface = false;  // per-node omega from tau(psi)
+    T taur = T(0), taub = T(0);
+    T grav = T(0);       // body force F_y = -grav rr
+};
+
+template <class T>
 struct AdjNode {
     T rr, rb, rho, aR, aB, ux, uy, uz, usq;
     T P[6];
@@ -153,14 +165,17 @@ struct AdjNode {
     T tr;  // -(Pxx + Pyy + Pzz)/3
     T Fx, Fy, Fz, absF, hA, sB, s2, rc;
     T q2, q3, beta;
+    T omega, sf, Fb, gy, g9, g3, e0;  // relaxation, body force, e0 with the force's u-only term
+    AdjExt<T> ext;
     bool bulk, cut;
 };
 
 // Forward set-up of node y from its moments, its colour gradient and the
 // parameters (A, beta, omega).
 template <class T>
-MCG_HD AdjNode<T> adj_setup(const T m[11], T Fx, T Fy, T Fz, T A, T beta, T omega) {
+MCG_HD AdjNode<T> adj_setup(const T m[11], T Fx, T Fy, T Fz, T A, T beta, T omega, const AdjExt<T> &ext = AdjExt<T>{}) {
     AdjNode<T> n;
+    n.ext = ext;
     n.rr = m[0];
     n.rb = m[1];
     n.rho = n.rr + n.rb;
@@ -180,10 +195,21 @@ MCG_HD AdjNode<T> adj_setup(const T m[11], T Fx, T Fy, T Fz, T A, T beta, T omeg
     n.t[TYYZ] = n.P[PYY] * n.uz + T(2) * n.P[PYZ] * n.uy;
     n.t[TYZZ] = n.P[PZZ] * n.uy + T(2) * n.P[PYZ] * n.uz;
     n.t[TXYZ] = T(2) * (n.P[PXY] * n.uz + n.P[PXZ] * n.uy + n.P[PYZ] * n.ux);
+    if (ext.iface) {
+        const T p = (n.rr - n.rb) / (n.rr + n.rb);
+        omega = T(1) / (((T(1) + p) / T(2)) * ext.taur + ((T(1) - p) / T(2)) * ext.taub);
+    }
+    n.omega = omega;
     const T oms = T(1) - omega;
     n.q2 = T(4.5) * oms;
     n.q3 = T(13.5) * oms;
     n.beta = beta;
+    n.sf = T(0.5) + T(0.5) * oms;
+    n.Fb = -ext.grav * n.rr;
+    n.gy = n.sf * n.Fb;
+    n.g9 = T(9) * n.gy;
+    n.g3 = T(1.5) * n.Fb;
+    n.e0 = n.rho * (T(1) - T(1.5) * n.usq) - T(3) * n.gy * n.uy;
     n.bulk = T(4) * n.rr * n.rb * inv * inv <= T(1.0e-4);
     n.Fx = n.Fy = n.Fz = n.absF = n.hA = n.sB = n.s2 = n.rc = T(0);
     n.cut = true;
@@ -226,7 +252,7 @@ MCG_HD T adj_a3(const AdjNode<T> &n, int i) {
 // forward pass).
 template <class T>
 MCG_HD void adj_collide(const AdjNode<T> &n, T *out_r, T *out_b) {
-    const T e0 = n.rho * (T(1) - T(1.5) * n.usq);
+    const T e0 = n.e0;
     {
         const T g0 = wT<T>(0) * (e0 + n.q2 * n.tr + T(1.25) * n.sB);
         out_r[0] = n.aR * g0;
@@ -236,8 +262,9 @@ MCG_HD void adj_collide(const AdjNode<T> &n, T *out_r, T *out_b) {
         const int i = 2 * k + 1;
         const T cu = T(kCX[i]) * n.ux + T(kCY[i]) * n.uy + T(kCZ[i]) * n.uz;
         const T d = T(kCX[i]) * n.Fx + T(kCY[i]) * n.Fy + T(kCZ[i]) * n.Fz;
-        const T E = e0 + T(4.5) * n.rho * cu * cu + n.q2 * adj_a2(n, i) + n.s2 * d * d - n.sB;
-        const T O = n.rho * cu * (T(3) - T(4.5) * n.usq) + T(4.5) * n.rho * cu * cu * cu + n.q3 * adj_a3(n, i);
+        const T E = e0 + T(4.5) * n.rho * cu * cu + n.q2 * adj_a2(n, i) + n.s2 * d * d - n.sB + T(kCY[i]) * n.g9 * cu;
+        const T O = n.rho * cu * (T(3) - T(4.5) * n.usq) + T(4.5) * n.rho * cu * cu * cu + n.q3 * adj_a3(n, i) +
+                    T(kCY[i]) * n.g3;
         const T D = n.rc * invcT<T>(i) * d;
         const T wi = wT<T>(i);
         const T gp = wi * (E + O), gm = wi * (E - O);
@@ -255,7 +282,7 @@ template <class T>
 struct AdjOut {
     T m[11];
     T F[3];
-    T A, beta;
+    T A, beta, grav;
 };
 
 // mu(i, r, b) yields the adjoint of the two colours of direction i; it is
@@ -265,7 +292,8 @@ template <class T, class MuFn>
 MCG_HD AdjOut<T> adj_vjp_fn(const AdjNode<T> &n, MuFn &&mu) {
     T l_rho = 0, l_aR = 0, l_aB = 0, l_usq = 0, l_sB = 0, l_s2 = 0, l_rc = 0, l_tr = 0;
     T l_u[3] = {0, 0, 0}, l_P[6] = {0, 0, 0, 0, 0, 0}, l_t[7] = {0, 0, 0, 0, 0, 0, 0}, l_F[3] = {0, 0, 0};
-    const T e0 = n.rho * (T(1) - T(1.5) * n.usq);
+    T l_e0 = 0, l_q2 = 0, l_q3 = 0, l_g9 = 0, l_g3 = 0;  // into the extension terms
+    const T e0 = n.e0;
     // rest direction
     {
         const T g0 = wT<T>(0) * (e0 + n.q2 * n.tr + T(1.25) * n.sB);
@@ -275,9 +303,9 @@ MCG_HD AdjOut<T> adj_vjp_fn(const AdjNode<T> &n, MuFn &&mu) {
         l_aR += g0 * mr0;
         l_aB += g0 * mb0;
         const T l_E0 = wT<T>(0) * l_g0;
-        l_rho += (T(1) - T(1.5) * n.usq) * l_E0;
-        l_usq += -T(1.5) * n.rho * l_E0;
+        l_e0 += l_E0;
         l_tr += n.q2 * l_E0;
+        l_q2 += n.tr * l_E0;
         l_sB += T(1.25) * l_E0;
     }
     for (int k = 0; k < 13; ++k) {
@@ -286,8 +314,8 @@ MCG_HD AdjOut<T> adj_vjp_fn(const AdjNode<T> &n, MuFn &&mu) {
         const T cu = cx * n.ux + cy * n.uy + cz * n.uz;
         const T d = cx * n.Fx + cy * n.Fy + cz * n.Fz;
         const T a2 = adj_a2(n, i), a3 = adj_a3(n, i);
-        const T E = e0 + T(4.5) * n.rho * cu * cu + n.q2 * a2 + n.s2 * d * d - n.sB;
-        const T O = n.rho * cu * (T(3) - T(4.5) * n.usq) + T(4.5) * n.rho * cu * cu * cu + n.q3 * a3;
+        const T E = e0 + T(4.5) * n.rho * cu * cu + n.q2 * a2 + n.s2 * d * d - n.sB + cy * n.g9 * cu;
+        const T O = n.rho * cu * (T(3) - T(4.5) * n.usq) + T(4.5) * n.rho * cu * cu * cu + n.q3 * a3 + cy * n.g3;
         const T wi = wT<T>(i);
         const T gp = wi * (E + O), gm = wi * (E - O);
         T mri, mbi, mrj, mbj;
@@ -301,10 +329,12 @@ MCG_HD AdjOut<T> adj_vjp_fn(const AdjNode<T> &n, MuFn &&mu) {
         const T l_E = wi * (l_gp + l_gm);
         const T l_O = wi * (l_gp - l_gm);
         // E
-        l_rho += ((T(1) - T(1.5) * n.usq) + T(4.5) * cu * cu) * l_E;
-        l_usq += -T(1.5) * n.rho * l_E;
-        T l_cu = T(9) * n.rho * cu * l_E;
+        l_e0 += l_E;
+        l_rho += T(4.5) * cu * cu * l_E;
+        T l_cu = T(9) * n.rho * cu * l_E + cy * n.g9 * l_E;
+        l_g9 += cy * cu * l_E;
         const T l_a2 = n.q2 * l_E;
+        l_q2 += a2 * l_E;
         l_s2 += d * d * l_E;
         T l_d = T(2) * n.s2 * d * l_E;
         l_sB += -l_E;
@@ -313,6 +343,8 @@ MCG_HD AdjOut<T> adj_vjp_fn(const AdjNode<T> &n, MuFn &&mu) {
         l_cu += n.rho * (T(3) - T(4.5) * n.usq + T(13.5) * cu * cu) * l_O;
         l_usq += -T(4.5) * n.rho * cu * l_O;
         const T l_a3 = n.q3 * l_O;
+        l_q3 += a3 * l_O;
+        l_g3 += cy * l_O;
         // D
         l_rc += invcT<T>(i) * d * l_D;
         l_d += n.rc * invcT<T>(i) * l_D;
@@ -334,9 +366,26 @@ MCG_HD AdjOut<T> adj_vjp_fn(const AdjNode<T> &n, MuFn &&mu) {
         for (int q = 0; q < 7; ++q) l_t[q] += h3T<T>(q, i) * l_a3;
     }
     AdjOut<T> o;
-    o.A = o.beta = T(0);
-    // interface coefficients: sB = hA |F|, s2 = hA/|F|, rc = beta rr rb/(rho |F|)
+    o.A = o.beta = o.grav = T(0);
     T l_rr = 0, l_rb = 0;
+    // e0 = rho (1 - 1.5 usq) - 3 gy u_y
+    l_rho += (T(1) - T(1.5) * n.usq) * l_e0;
+    l_usq += -T(1.5) * n.rho * l_e0;
+    T l_gy = -T(3) * n.uy * l_e0 + T(9) * l_g9;
+    l_u[1] += -T(3) * n.gy * l_e0;
+    // gy = sf Fb, g3 = 1.5 Fb, Fb = -grav rr, sf = 1 - omega/2, q2 = 4.5 (1 - omega), q3 = 13.5 (1 - omega)
+    const T l_Fb = n.sf * l_gy + T(1.5) * l_g3;
+    const T l_sf = n.Fb * l_gy;
+    l_rr += -n.ext.grav * l_Fb;
+    o.grav += -n.rr * l_Fb;
+    if (n.ext.iface) {
+        // omega = 1/tau(psi): d omega/d psi = -omega^2 (taur - taub)/2, psi = (rr - rb)/rho
+        const T l_om = -T(0.5) * l_sf - T(4.5) * l_q2 - T(13.5) * l_q3;
+        const T dpsi = -n.omega * n.omega * (n.ext.taur - n.ext.taub) / T(2) * l_om;
+        l_rr += dpsi * T(2) * n.rb / (n.rho * n.rho);
+        l_rb += -dpsi * T(2) * n.rr / (n.rho * n.rho);
+    }
+    // interface coefficients: sB = hA |F|, s2 = hA/|F|, rc = beta rr rb/(rho |F|)
     if (!n.bulk) {
         T l_absF = n.hA * l_sB;
         o.A += n.absF * l_sB / T(2);
@@ -399,7 +448,7 @@ MCG_HD AdjOut<T> adj_vjp(const AdjNode<T> &n, const T *mu_r, const T *mu_b) {
 // m and their adjoint lam (11) to the adjoint of the 54 gathered populations.
 //   rr = sum f_r, rb = sum f_b, u = sum g c / rho, Pi_ab = sum g H_ab - rho u_a u_b
 template <class T>
-MCG_HD void adj_moments(const T m[11], const T lam[11], T *l_fr, T *l_fb) {
+MCG_HD void adj_moments(const T m[11], const T lam[11], T *l_fr, T *l_fb, T ghalf = T(0), T *dgrav = nullptr) {
     const T rho = m[0] + m[1];
     const T u[3] = {m[2], m[3], m[4]};
     // Pi_ab = S_ab - rho u_a u_b  ->  l_S = l_Pi, l_u, l_rho
@@ -420,7 +469,9 @@ MCG_HD void adj_moments(const T m[11], const T lam[11], T *l_fr, T *l_fb) {
     const T inv = T(1) / rho;
     const T l_j[3] = {l_u[0] * inv, l_u[1] * inv, l_u[2] * inv};
     l_rho += -(l_u[0] * u[0] + l_u[1] * u[1] + l_u[2] * u[2]) * inv;
-    const T l_rr = lam[0] + l_rho, l_rb = lam[1] + l_rho;
+    // u_y = (j_y - ghalf rr)/rho, ghalf = grav/2: the explicit rr and grav dependence of the force shift
+    const T l_rr = lam[0] + l_rho - ghalf * l_u[1] * inv, l_rb = lam[1] + l_rho;
+    if (dgrav) *dgrav += -T(0.5) * m[0] * inv * l_u[1];
     for (int i = 0; i < Q; ++i) {
         T lg = T(kCX[i]) * l_j[0] + T(kCY[i]) * l_j[1] + T(kCZ[i]) * l_j[2];
         for (int k = 0; k < 6; ++k) lg += h2T<T>(k, i) * lP[k];

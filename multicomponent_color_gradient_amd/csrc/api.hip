@@ -910,12 +910,14 @@ int mcg_device_bytes(const mcg_solver *s, uint64_t *nbytes) {
 static int adj_supported(const mcg_case &c, int storage) {
     if (c.y_mode == MCG_Y_JET) {
         if (c.wrap_gradient) return fail(MCG_E_UNSUPPORTED, "the jet adjoint covers the reference gradient (wrap_gradient 0)");
-        if (c.sponge_K != 0.0) return fail(MCG_E_UNSUPPORTED, "the jet adjoint assumes a uniform relaxation rate (sponge_K = 0)");
-    } else if (c.y_mode != MCG_Y_PERIODIC || !c.wrap_gradient) {
-        return fail(MCG_E_UNSUPPORTED, "the adjoint covers jet cases and the periodic domain (y_mode periodic, wrap_gradient 1)");
+        if (c.viscosity_mode != MCG_VISC_SINGLE) return fail(MCG_E_UNSUPPORTED, "the jet adjoint covers the single relaxation rate");
+        if (c.gravity != 0.0) return fail(MCG_E_UNSUPPORTED, "the jet adjoint does not cover body forces");
+    } else if (!c.wrap_gradient) {
+        return fail(MCG_E_UNSUPPORTED, "the periodic adjoint needs wrap_gradient 1 (the gradient wraps the periodic axes)");
     }
-    if (c.viscosity_mode != MCG_VISC_SINGLE) return fail(MCG_E_UNSUPPORTED, "the adjoint covers the single relaxation rate");
-    if (c.wall_xz || c.gravity != 0.0) return fail(MCG_E_UNSUPPORTED, "the adjoint does not cover walls or body forces");
+    if (c.sponge_K != 0.0) return fail(MCG_E_UNSUPPORTED, "the adjoint assumes no sponge (sponge_K = 0)");
+    if (c.viscosity_mode == MCG_VISC_INTERFACE && !(c.A_override > 0))
+        return fail(MCG_E_UNSUPPORTED, "the adjoint with per-node viscosity needs A given directly (A_override > 0)");
     if (storage != MCG_STORE_FP32) return fail(MCG_E_UNSUPPORTED, "the adjoint runs on FP32 populations");
     return MCG_OK;
 }
@@ -1252,14 +1254,14 @@ int mcg_adjoint_release(mcg_solver *s) {
     return MCG_OK;
 }
 
-int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *lam_fr,
-                    const double *lam_fb, double *out_fr, double *out_fb, double *out_lam_fr, double *out_lam_fb,
-                    double *dparams) {
+int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const double *fb0, const double *fields0,
+                    const double *lam_fr, const double *lam_fb, double *out_fr, double *out_fb, double *out_lam_fr,
+                    double *out_lam_fb, double *dparams) {
     if (!c || !fr0 || !fb0) return fail(MCG_E_INVALID, "null pointer");
     if (int rc = mcg_case_validate(c)) return rc;
     if (c->y_mode != MCG_Y_PERIODIC) return fail(MCG_E_UNSUPPORTED, "jet cases: use mcg_cpu_adjoint_jet");
     if (int rc = adj_supported(*c, MCG_STORE_FP32)) return rc;
-    if (c->sponge_K != 0.0) return fail(MCG_E_UNSUPPORTED, "the CPU adjoint assumes a uniform relaxation rate (sponge_K = 0)");
+    if (c->wall_xz && !fields0) return fail(MCG_E_INVALID, "walls need fields0 (the ghost densities)");
     if (nsteps < 0) return fail(MCG_E_INVALID, "nsteps < 0");
     CpuAdjoint A;
     A.NX = c->NX;
@@ -1269,6 +1271,15 @@ int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const doub
     A.omega = 1.0 / c->taub;
     A.A = c->A_override > 0.0 ? c->A_override : c->sigma / (4.0 / 9.0) / c->taub;
     A.beta = c->beta_recolor;
+    A.wall = c->wall_xz != 0;
+    A.ext.iface = c->viscosity_mode == MCG_VISC_INTERFACE;
+    A.ext.taur = c->taur;
+    A.ext.taub = c->taub;
+    A.ext.grav = c->gravity;
+    if (A.wall) {
+        A.gpsi.resize(A.N);
+        for (size_t g = 0; g < A.N; ++g) A.gpsi[g] = (fields0[g] - fields0[A.N + g]) / (fields0[g] + fields0[A.N + g]);
+    }
     const size_t n = size_t(Q) * A.N;
     std::vector<std::vector<double>> R(size_t(nsteps) + 1, std::vector<double>(n)), B(size_t(nsteps) + 1, std::vector<double>(n));
     std::copy(fr0, fr0 + n, R[0].begin());
@@ -1278,9 +1289,9 @@ int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const doub
     if (out_fb) std::copy(B[nsteps].begin(), B[nsteps].end(), out_fb);
     if (!lam_fr || !lam_fb) return MCG_OK;
     std::vector<double> lr(lam_fr, lam_fr + n), lb(lam_fb, lam_fb + n), tr(n), tb(n);
-    double dA = 0.0, dbeta = 0.0;
+    double dA = 0.0, dbeta = 0.0, dgrav = 0.0;
     for (int k = nsteps - 1; k >= 0; --k) {
-        A.adjoint(R[k].data(), B[k].data(), lr.data(), lb.data(), tr.data(), tb.data(), dA, dbeta);
+        A.adjoint(R[k].data(), B[k].data(), lr.data(), lb.data(), tr.data(), tb.data(), dA, dbeta, dgrav);
         lr.swap(tr);
         lb.swap(tb);
     }
@@ -1289,6 +1300,7 @@ int mcg_cpu_adjoint(const mcg_case *c, int nsteps, const double *fr0, const doub
     if (dparams) {
         dparams[0] = dA;
         dparams[1] = dbeta;
+        dparams[2] = dgrav;
     }
     return MCG_OK;
 }
