@@ -233,10 +233,12 @@ int mcg_checkpoint_save(mcg_solver *s, void *host_buf, uint64_t nbytes);
 int mcg_checkpoint_load(mcg_solver *s, const void *host_buf, uint64_t nbytes);
 
 /* ---- adjoint (extension: the reference has no adjoint) ---------------- */
-/* Discrete adjoint of the step, single relaxation rate, FP32 populations, for
- * the periodic domain (y_mode periodic, wrap_gradient 1) and for jet cases
- * (y_mode jet, wrap_gradient 0, sponge_K 0: the inlet and the convective
- * outlet are differentiated and the outlet state is carried between steps).
+/* Discrete adjoint of the step, FP32 populations, sponge_K 0, for the
+ * periodic-y family (y_mode periodic, wrap_gradient 1; also with x/z walls,
+ * the body force and the per-node viscosity, which needs A_override > 0)
+ * and for jet cases (y_mode jet, wrap_gradient 0, single relaxation rate: the
+ * inlet and the convective outlet are differentiated and the outlet state is
+ * carried between steps).
  * The state is the populations in the reference layout (what
  * mcg_copy_populations returns: f_i(x) after streaming); the parameters are A
  * (the perturbation amplitude, A_override or sigma/(4 cs^4 taub)) and beta,
@@ -290,6 +292,12 @@ int mcg_adjoint_inlet_gradient(const mcg_solver *s, double *dparams, double *din
  * the final state is restored. */
 int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb, float *out_lam_fr, float *out_lam_fb,
                          double *dparams);
+
+/* The FP64 parameter gradients of the last mcg_adjoint_backward sweep: the
+ * first n (<= 5) of {dJ/dA, dJ/dbeta, dJ/dgravity, dJ/djet_velocity,
+ * dJ/dperturbation_amp} (the last two are zero outside jet cases).  No
+ * counterpart in the reference (an extension). */
+int mcg_adjoint_gradients(const mcg_solver *s, double *out, int n);
 
 /* Device milliseconds of the last mcg_adjoint_backward sweep (segment
  * recomputation and adjoint kernels, HIP events on the solver's stream;

@@ -57,17 +57,24 @@ struct AdjJet {
     const float *w_uy;  // seed on u_y (full grid) or null
 };
 
-// Jet cases: psi at every position of the full grid as the colour gradient
-// reads it (forces.cuh:31-33, no wrap): interior nodes from the moments, the
+// Adjoint modes: periodic domain, jet (inlet/outlet planes), x/z walls with
+// periodic y.
+enum { kAdjPeriodic = 0, kAdjJet = 1, kAdjWall = 2 };
+
+// psi at every position of the full grid as the colour gradient reads it.
+// Jet cases (forces.cuh:31-33, no wrap): interior nodes from the moments, the
 // inner columns of the outlet plane NY-1 from this step's outlet state gBn,
-// every other ghost position from the ghost densities.
+// every other ghost position from the ghost densities.  Walls: the same with
+// the y ghost planes holding the wrapped planes NY-2 and 1.
 __global__ __launch_bounds__(256) void k_psi_full(DevParams P, const float *__restrict__ mom, const float *__restrict__ gBn,
                                                   const float *__restrict__ grr, const float *__restrict__ grb,
                                                   float *__restrict__ psf) {
     const int x = int(blockIdx.x) * 64 + int(threadIdx.x);
-    const int y = int(blockIdx.y) * 4 + int(threadIdx.y);
+    const int yo = int(blockIdx.y) * 4 + int(threadIdx.y);
     const int z = int(blockIdx.z);
-    if (x >= P.NX || y >= P.NY) return;
+    if (x >= P.NX || yo >= P.NY) return;
+    const size_t go = gidx(P, x, yo, z);
+    const int y = P.wall ? wrapN(yo, P.NY) : yo;
     const size_t g = gidx(P, x, y, z);
     const bool inner = x >= 1 && x <= P.NX - 2 && z >= 1 && z <= P.NZ - 2;
     float v;
@@ -79,28 +86,30 @@ __global__ __launch_bounds__(256) void k_psi_full(DevParams P, const float *__re
     } else {
         v = psi(grr[g], grb[g]);
     }
-    psf[g] = v;
+    psf[go] = v;
 }
 
 // Adjoint of the collision at every interior node: lam_m (11 planes),
-// lam_F (3 planes), and the workgroup's sums of dJ/dA, dJ/dbeta in
-// part[2 b], part[2 b + 1] (b = linear workgroup id; reduced by k_adj_sum in
+// lam_F (3 planes), and the workgroup's sums of dJ/dA, dJ/dbeta, dJ/dgravity
+// in part[3 b .. 3 b + 2] (b = linear workgroup id; reduced by k_adj_sum in
 // a fixed order, so the FP64 parameter gradients are deterministic).
-// Jet cases (JET): the colour gradient reads psi without wrap, ghost
-// positions from the ghost densities except the outlet plane NY-1 (its inner
-// columns hold the outlet state gBn of this step); populations that stream
-// through the planes y=0 / y=NY-1 leave the domain; w_uy (optional) seeds u_y.
-template <bool JET>
+// Jet cases: the colour gradient reads psi without wrap, ghost positions from
+// the ghost densities except the outlet plane NY-1 (its inner columns hold
+// the outlet state gBn of this step); populations that stream through the
+// planes y=0 / y=NY-1 leave the domain; w_uy (optional) seeds u_y.  Walls:
+// populations streaming into an x/z wall come back to the node in the
+// opposite direction; the gradient reads the frozen ghost psi there.
+template <int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MCG_ADJ_WAVES_PER_EU)))
 void k_adj_node(DevParams P, const float *__restrict__ mom, const float *__restrict__ ps,
                 const float *__restrict__ lam_in, float *__restrict__ lam_m, float *__restrict__ lam_F, double *part,
                 AdjJet J) {
-    __shared__ double wsum[2][4];
+    __shared__ double wsum[3][4];
     const int x = 1 + int(blockIdx.x) * 64 + int(threadIdx.x);
     const int y = 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
     const int z = 1 + int(blockIdx.z);
     const bool live = x <= P.NX - 2 && y <= P.NY - 2;
-    double gA = 0.0, gB = 0.0;
+    double gA = 0.0, gB = 0.0, gG = 0.0;
     if (live) {
         const uint32_t o = midx(P, x, y, z);
         float m[11];
@@ -108,74 +117,98 @@ void k_adj_node(DevParams P, const float *__restrict__ mom, const float *__restr
         float Fx = 0.f, Fy = 0.f, Fz = 0.f;
         auto phi = [&](auto Ic) -> float {
             constexpr int I = decltype(Ic)::value;
-            if constexpr (JET) return J.psf[gidx(P, x + kCX[I], y + kCY[I], z + kCZ[I])];
+            if constexpr (MODE != kAdjPeriodic) return J.psf[gidx(P, x + kCX[I], y + kCY[I], z + kCZ[I])];
             return ps[midx(P, wrapN(x + kCX[I], P.NX), wrapN(y + kCY[I], P.NY), wrapN(z + kCZ[I], P.NZ))];
         };
         color_gradient(phi, Fx, Fy, Fz);
         const float omega = omega_node(P, y, m[0], m[1]);
-        const AdjNode<float> n = adj_setup<float>(m, Fx, Fy, Fz, A_node(P, m[0], m[1]), P.beta, omega);
+        AdjExt<float> ext;
+        ext.iface = P.visc_mode == MCG_VISC_INTERFACE;
+        ext.taur = P.taur;
+        ext.taub = P.taub;
+        ext.grav = P.grav;
+        const AdjNode<float> n = adj_setup<float>(m, Fx, Fy, Fz, A_node(P, m[0], m[1]), P.beta, omega, ext);
+        const uint32_t self = pidx(P, x, y, z);
         // mu_i(x) = lambda_i(x + c_i), loaded where the pair needs it
         const AdjOut<float> a = adj_vjp_fn<float>(n, [&](int i, float &r, float &b) {
-            const int ty = JET ? y + kCY[i] : wrapN(y + kCY[i], P.NY);
-            if (JET && (ty < 1 || ty > P.NY - 2)) {  // leaves through the inlet or outlet plane
-                r = b = 0.0f;
-                return;
+            uint32_t q;
+            int row = i;
+            if constexpr (MODE == kAdjJet) {
+                const int ty = y + kCY[i];
+                if (ty < 1 || ty > P.NY - 2) {  // leaves through the inlet or outlet plane
+                    r = b = 0.0f;
+                    return;
+                }
+                q = pidx(P, wrapN(x + kCX[i], P.NX), ty, wrapN(z + kCZ[i], P.NZ));
+            } else if constexpr (MODE == kAdjWall) {
+                const int tx = x + kCX[i], tz = z + kCZ[i];
+                if (tx < 1 || tx > P.NX - 2 || tz < 1 || tz > P.NZ - 2) {  // bounced back into direction opp(i)
+                    q = self;
+                    row = opp(i);
+                } else {
+                    q = pidx(P, tx, wrapN(y + kCY[i], P.NY), tz);
+                }
+            } else {
+                q = pidx(P, wrapN(x + kCX[i], P.NX), wrapN(y + kCY[i], P.NY), wrapN(z + kCZ[i], P.NZ));
             }
-            const uint32_t q = pidx(P, wrapN(x + kCX[i], P.NX), ty, wrapN(z + kCZ[i], P.NZ));
-            r = lam_in[uint32_t(i) * uint32_t(P.PX) + q];
-            b = lam_in[uint32_t(Q + i) * uint32_t(P.PX) + q];
+            r = lam_in[uint32_t(row) * uint32_t(P.PX) + q];
+            b = lam_in[uint32_t(Q + row) * uint32_t(P.PX) + q];
         });
-        for (int k = 0; k < 11; ++k) lam_m[size_t(k) * P.plane + o] = a.m[k];
-        if (JET && J.w_uy) lam_m[3 * size_t(P.plane) + o] += J.w_uy[gidx(P, x, y, z)];
+        float lm[11];
+        for (int k = 0; k < 11; ++k) lm[k] = a.m[k];
+        if (MODE == kAdjJet && J.w_uy) lm[3] += J.w_uy[gidx(P, x, y, z)];
+        for (int k = 0; k < 11; ++k) lam_m[size_t(k) * P.plane + o] = lm[k];
         for (int k = 0; k < 3; ++k) lam_F[size_t(k) * P.plane + o] = a.F[k];
         gA = double(a.A);
         gB = double(a.beta);
+        gG = double(a.grav);
+        if (P.grav != 0.0f) {
+            // the moments pass: u_y = (j_y - grav rr/2)/rho; the adjoint of u_y includes Pi's share
+            const float rho = m[0] + m[1];
+            const float luy = lm[3] - 2.0f * rho * m[3] * lm[5 + PYY] - rho * (m[2] * lm[5 + PXY] + m[4] * lm[5 + PYZ]);
+            gG += -0.5 * double(m[0]) / double(rho) * double(luy);
+        }
     }
     gA = wave_sum(gA);
     gB = wave_sum(gB);
+    gG = wave_sum(gG);
     if (threadIdx.x == 0) {
         wsum[0][threadIdx.y] = gA;
         wsum[1][threadIdx.y] = gB;
+        wsum[2][threadIdx.y] = gG;
     }
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         const size_t b = size_t(blockIdx.x) + size_t(gridDim.x) * (size_t(blockIdx.y) + size_t(gridDim.y) * blockIdx.z);
-        part[2 * b] = (wsum[0][0] + wsum[0][1]) + (wsum[0][2] + wsum[0][3]);
-        part[2 * b + 1] = (wsum[1][0] + wsum[1][1]) + (wsum[1][2] + wsum[1][3]);
+        for (int v = 0; v < 3; ++v) part[3 * b + v] = (wsum[v][0] + wsum[v][1]) + (wsum[v][2] + wsum[v][3]);
     }
 }
 
-// acc[0..1] += the sums of the nb workgroup partials (one workgroup of 1024
-// threads, four independent accumulators each, fixed order: deterministic)
+// acc[0..NV-1] += the sums of the nb workgroup partials part[NV b + v] (one
+// workgroup of 1024 threads, four independent accumulators each, fixed
+// order: deterministic)
 constexpr int kSumThreads = 1024;
+template <int NV>
 __global__ __launch_bounds__(kSumThreads) void k_adj_sum(const double *__restrict__ part, size_t nb, double *acc) {
-    __shared__ double sh[2][kSumThreads];
-    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+    __shared__ double sh[NV][kSumThreads];
+    double a[NV][4];
+    for (int v = 0; v < NV; ++v) a[v][0] = a[v][1] = a[v][2] = a[v][3] = 0.0;
     size_t k = threadIdx.x;
     for (; k + 3 * kSumThreads < nb; k += 4 * kSumThreads) {
-        for (int u = 0; u < 4; ++u) {
-            a[u] += part[2 * (k + u * kSumThreads)];
-            b[u] += part[2 * (k + u * kSumThreads) + 1];
-        }
+        for (int u = 0; u < 4; ++u)
+            for (int v = 0; v < NV; ++v) a[v][u] += part[NV * (k + u * kSumThreads) + v];
     }
-    for (; k < nb; k += kSumThreads) {
-        a[0] += part[2 * k];
-        b[0] += part[2 * k + 1];
-    }
-    sh[0][threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
-    sh[1][threadIdx.x] = (b[0] + b[1]) + (b[2] + b[3]);
+    for (; k < nb; k += kSumThreads)
+        for (int v = 0; v < NV; ++v) a[v][0] += part[NV * k + v];
+    for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] = (a[v][0] + a[v][1]) + (a[v][2] + a[v][3]);
     __syncthreads();
     for (int h = kSumThreads / 2; h > 0; h >>= 1) {
-        if (int(threadIdx.x) < h) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + h];
-        }
+        if (int(threadIdx.x) < h)
+            for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] += sh[v][threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        acc[0] += sh[0][0];
-        acc[1] += sh[1][0];
-    }
+    if (threadIdx.x == 0)
+        for (int v = 0; v < NV; ++v) acc[v] += sh[v][0];
 }
 
 // Per-workgroup partial sums of w(x) u_y(x) over the interior nodes (the
@@ -203,7 +236,7 @@ __global__ __launch_bounds__(256) void k_uy_partial(DevParams P, const float *__
 // lambda^n at every interior node.  Jet cases: the transpose runs over the
 // interior neighbours without wrap, and the stale inlet/outlet slots (never
 // rewritten, so carried unchanged) pass their adjoint through.
-template <bool JET>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_adj_pop(DevParams P, const float *__restrict__ mom,
                                                  const float *__restrict__ lam_m, const float *__restrict__ lam_F,
                                                  const float *__restrict__ lam_in, float *__restrict__ lam_out) {
@@ -213,8 +246,11 @@ __global__ __launch_bounds__(256) void k_adj_pop(DevParams P, const float *__res
     unroll<1, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
         int sx = x - kCX[I], sy = y - kCY[I], sz = z - kCZ[I];
-        if constexpr (JET) {
+        if constexpr (MODE == kAdjJet) {
             if (sx < 1 || sx > P.NX - 2 || sy < 1 || sy > P.NY - 2 || sz < 1 || sz > P.NZ - 2) return;
+        } else if constexpr (MODE == kAdjWall) {
+            if (sx < 1 || sx > P.NX - 2 || sz < 1 || sz > P.NZ - 2) return;  // ghost psi is constant
+            sy = wrapN(sy, P.NY);
         } else {
             sx = wrapN(sx, P.NX);
             sy = wrapN(sy, P.NY);
@@ -237,8 +273,8 @@ __global__ __launch_bounds__(256) void k_adj_pop(DevParams P, const float *__res
     lam[0] += Lpsi * 2.0f * m[1] * inv2;
     lam[1] += -Lpsi * 2.0f * m[0] * inv2;
     float lr[Q], lb[Q];
-    adj_moments<float>(m, lam, lr, lb);
-    if constexpr (JET) {
+    adj_moments<float>(m, lam, lr, lb, P.ghalf);
+    if constexpr (MODE == kAdjJet) {
         auto inner = [&](int a, int c) { return a >= 1 && a <= P.NX - 2 && c >= 1 && c <= P.NZ - 2; };
         for (int i = 0; i < Q; ++i) {
             const int gx = x - kCX[i], gz = z - kCZ[i];

@@ -116,6 +116,7 @@ struct mcg_solver {
     float *adj_wuy = nullptr;
     int64_t adj_uy_first = 0, adj_uy_last = -1;  // solver step numbers
     double adj_inlet[2] = {0.0, 0.0};  // dJ/djet_velocity, dJ/dperturbation_amp of the last sweep
+    double adj_grad[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // dJ/d{A, beta, gravity, jet_velocity, perturbation_amp}
 
     void adj_release() {
         for (void *p : adj_ckpt) (void)hipFree(p);
@@ -391,7 +392,7 @@ int one_step(mcg_solver *s) {
         const dim3 g = s->grid_interior();
         hipLaunchKernelGGL(k_uy_partial, g, dim3(64, 4, 1), 0, s->stream, s->P, s->mom, s->adj_wuy, s->obs_part);
         LAUNCH_CHECK("k_uy_partial");
-        hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(kSumThreads), 0, s->stream, s->obs_part, size_t(g.x) * g.y * g.z,
+        hipLaunchKernelGGL(k_adj_sum<2>, dim3(1), dim3(kSumThreads), 0, s->stream, s->obs_part, size_t(g.x) * g.y * g.z,
                            s->obs_acc);
         LAUNCH_CHECK("k_adj_sum");
     }
@@ -1067,8 +1068,8 @@ static int adj_alloc(mcg_solver *s) {
         ok = ok && get(&s->adj_pp[b], s->state_bytes());
         if (s->jet()) ok = ok && get((void **)&s->adj_lg[b], s->gB_bytes());
     }
-    if (s->jet()) {
-        ok = ok && get((void **)&s->adj_din, 3 * s->nxz() * sizeof(double));
+    if (s->jet()) ok = ok && get((void **)&s->adj_din, 3 * s->nxz() * sizeof(double));
+    if (s->jet() || s->c.wall_xz) {
         ok = ok && get((void **)&s->adj_psf, s->N() * sizeof(float));
         ok = ok && get((void **)&s->adj_gbs, s->gB_set_bytes() * seg_len);
     } else {
@@ -1076,10 +1077,10 @@ static int adj_alloc(mcg_solver *s) {
     }
     ok = ok && get((void **)&s->adj_lm, size_t(11) * s->P.plane * sizeof(float));
     ok = ok && get((void **)&s->adj_lF, size_t(3) * s->P.plane * sizeof(float));
-    ok = ok && get((void **)&s->adj_acc, 4 * sizeof(double));
+    ok = ok && get((void **)&s->adj_acc, 5 * sizeof(double));
     ok = ok && get((void **)&s->adj_mom, s->mom_set_bytes() * seg_len);
     const dim3 g = s->grid_interior();
-    ok = ok && get((void **)&s->adj_part, 2 * sizeof(double) * size_t(g.x) * g.y * g.z);
+    ok = ok && get((void **)&s->adj_part, 3 * sizeof(double) * size_t(g.x) * g.y * g.z);
     return ok ? MCG_OK : fail(MCG_E_NOMEM, "adjoint buffer allocation failed");
 }
 
@@ -1089,7 +1090,9 @@ static int adj_alloc(mcg_solver *s) {
 static int adjoint_step(mcg_solver *s, int k, const float *gBn, const float *mom, float *lam_in, float *lam_out,
                         float *lg_in, float *lg_out) {
     const dim3 g = s->grid_interior();
+    const size_t nb = size_t(g.x) * g.y * g.z;
     AdjJet J{};
+    // node partials {A, beta, gravity} -> acc[0..2]; jet column partials {U, amp} -> acc[3..4]
     if (s->jet()) {
         hipLaunchKernelGGL(k_psi_full, s->grid_full(), dim3(64, 4, 1), 0, s->stream, s->P, mom, gBn, s->grr, s->grb,
                            s->adj_psf);
@@ -1097,32 +1100,42 @@ static int adjoint_step(mcg_solver *s, int k, const float *gBn, const float *mom
         J.psf = s->adj_psf;
         const int64_t n = s->adj_step0 + k;
         J.w_uy = (s->adj_wuy && n >= s->adj_uy_first && n <= s->adj_uy_last) ? s->adj_wuy : nullptr;
-        hipLaunchKernelGGL(k_adj_node<true>, g, dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_psf, lam_in, s->adj_lm,
+        hipLaunchKernelGGL(k_adj_node<kAdjJet>, g, dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_psf, lam_in, s->adj_lm,
                            s->adj_lF, s->adj_part, J);
         LAUNCH_CHECK("k_adj_node");
-        hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, size_t(g.x) * g.y * g.z,
-                           s->adj_acc);
+        hipLaunchKernelGGL(k_adj_sum<3>, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, nb, s->adj_acc);
         LAUNCH_CHECK("k_adj_sum");
         const int cols = (s->c.NX - 2) * (s->c.NZ - 2);
         const dim3 gc(unsigned((cols + 255) / 256));
         hipLaunchKernelGGL(k_adj_cols, gc, dim3(256), 0, s->stream, s->P, mom, lam_in, s->adj_lF, s->adj_lm, gBn, lg_in,
                            lg_out, s->adj_din, s->adj_part, (unsigned long long)(s->adj_step0 + k), s->inlet_du);
         LAUNCH_CHECK("k_adj_cols");
-        hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, size_t(gc.x), s->adj_acc + 2);
+        hipLaunchKernelGGL(k_adj_sum<2>, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, size_t(gc.x), s->adj_acc + 3);
         LAUNCH_CHECK("k_adj_sum");
-        hipLaunchKernelGGL(k_adj_pop<true>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_lm,
+        hipLaunchKernelGGL(k_adj_pop<kAdjJet>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_lm,
+                           s->adj_lF, lam_in, lam_out);
+    } else if (s->c.wall_xz) {
+        hipLaunchKernelGGL(k_psi_full, s->grid_full(), dim3(64, 4, 1), 0, s->stream, s->P, mom, nullptr, s->grr, s->grb,
+                           s->adj_psf);
+        LAUNCH_CHECK("k_psi_full");
+        J.psf = s->adj_psf;
+        hipLaunchKernelGGL(k_adj_node<kAdjWall>, g, dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_psf, lam_in,
+                           s->adj_lm, s->adj_lF, s->adj_part, J);
+        LAUNCH_CHECK("k_adj_node");
+        hipLaunchKernelGGL(k_adj_sum<3>, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, nb, s->adj_acc);
+        LAUNCH_CHECK("k_adj_sum");
+        hipLaunchKernelGGL(k_adj_pop<kAdjWall>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_lm,
                            s->adj_lF, lam_in, lam_out);
     } else {
         hipLaunchKernelGGL(k_psi_plane, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_ps);
         LAUNCH_CHECK("k_psi_plane");
-        hipLaunchKernelGGL(k_adj_node<false>, g, dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_ps, lam_in, s->adj_lm,
-                           s->adj_lF, s->adj_part, J);
+        hipLaunchKernelGGL(k_adj_node<kAdjPeriodic>, g, dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_ps, lam_in,
+                           s->adj_lm, s->adj_lF, s->adj_part, J);
         LAUNCH_CHECK("k_adj_node");
-        hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, size_t(g.x) * g.y * g.z,
-                           s->adj_acc);
+        hipLaunchKernelGGL(k_adj_sum<3>, dim3(1), dim3(kSumThreads), 0, s->stream, s->adj_part, nb, s->adj_acc);
         LAUNCH_CHECK("k_adj_sum");
-        hipLaunchKernelGGL(k_adj_pop<false>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, mom, s->adj_lm,
-                           s->adj_lF, lam_in, lam_out);
+        hipLaunchKernelGGL(k_adj_pop<kAdjPeriodic>, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, mom,
+                           s->adj_lm, s->adj_lF, lam_in, lam_out);
     }
     LAUNCH_CHECK("k_adj_pop");
     return MCG_OK;
@@ -1150,7 +1163,7 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     hipLaunchKernelGGL(k_rows_from_ref, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, s->staging, s->staging + n,
                        s->adj_lam[0]);
     LAUNCH_CHECK("k_rows_from_ref");
-    HIP_TRY(hipMemsetAsync(s->adj_acc, 0, 4 * sizeof(double), s->stream));
+    HIP_TRY(hipMemsetAsync(s->adj_acc, 0, 5 * sizeof(double), s->stream));
     if (s->jet()) {
         // stale inlet/outlet entries, identical in every recorded state
         for (void *p : s->adj_pp) {
@@ -1217,15 +1230,23 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     LAUNCH_CHECK("k_rows_to_ref");
     if (out_lam_fr) HIP_TRY(hipMemcpyAsync(out_lam_fr, s->staging, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     if (out_lam_fb) HIP_TRY(hipMemcpyAsync(out_lam_fb, s->staging + n, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     HIP_TRY(hipMemcpyAsync(acc, s->adj_acc, sizeof(acc), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    s->adj_inlet[0] = acc[2];
-    s->adj_inlet[1] = acc[3];
+    for (int v = 0; v < 5; ++v) s->adj_grad[v] = acc[v];
+    s->adj_inlet[0] = acc[3];
+    s->adj_inlet[1] = acc[4];
     if (dparams) {
         dparams[0] = acc[0];
         dparams[1] = acc[1];
     }
+    return MCG_OK;
+}
+
+int mcg_adjoint_gradients(const mcg_solver *s, double *out, int n) {
+    if (int rc = check_solver(s)) return rc;
+    if (!out || n < 0 || n > 5) return fail(MCG_E_INVALID, "need 0 <= n <= 5 and an output array");
+    for (int v = 0; v < n; ++v) out[v] = s->adj_grad[v];
     return MCG_OK;
 }
 

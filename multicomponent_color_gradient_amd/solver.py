@@ -178,6 +178,12 @@ class Solver:
         w = np.ascontiguousarray(w_uy, dtype=np.float32).reshape(self.shape)
         check(self.lib.mcg_adjoint_set_uy_seed(self._h, as_ptr(w, ctypes.c_float), int(first), int(last)))
 
+    def adjoint_gradients(self) -> dict:
+        """FP64 parameter gradients of the last adjoint_backward sweep."""
+        out = np.zeros(5, np.float64)
+        check(self.lib.mcg_adjoint_gradients(self._h, as_ptr(out, ctypes.c_double), 5))
+        return dict(zip(["dA", "dbeta", "dgravity", "dU", "damp"], out.tolist()))
+
     def uy_functional(self) -> float:
         """Value of the u_y functional accumulated since adjoint_set_uy_seed (FP64)."""
         v = ctypes.c_double()
