@@ -85,28 +85,28 @@ def copy_probe(nbytes: int, reps: int = 3) -> None:
 
 
 def adjoint_ratio(steps: int, every: int) -> dict:
-    """Adjoint/forward step-cost ratio on a 192x384x192 periodic bubble (R=16,
-    tau 0.8, A=1e-3, beta 0.7): device time of the backward sweep (checkpoint
-    recomputation + adjoint kernels) over the device time of the same number
-    of plain forward steps."""
-    import numpy as np
-
+    """Adjoint/forward step-cost ratio on the 192x384x192 rising bubble (walls in
+    x/z, periodic y, R=16 at y=64, gravity 1e-5 on red, tau 0.55/1.0, A=1e-3,
+    beta 0.7): device time of the backward sweep (checkpoint recomputation +
+    adjoint kernels) for the centroid-height objective over the device time of
+    the same number of plain forward steps."""
     import multicomponent_color_gradient_amd as m
+    from multicomponent_color_gradient_amd.rising_bubble import centroid_seed
 
-    c = m.periodic_bubble_case(192, radius=16.0, tau=0.8, A=1e-3, beta=0.7)
-    c.NY = 384
-    c.bubble_y0 = (c.NY - 1) / 2.0
+    c = m.rising_bubble_case(192, 384, 192)
     s = m.Solver(c)
     s.init_bubble_tanh()
     s.step(10)
     fwd_ms = s.time_steps(steps)
     s.adjoint_record(steps, every)
-    lam = np.zeros((27,) + s.shape, np.float32)
-    lam[:, 1:-1, 1:-1, 1:-1] = 1e-3
-    s.adjoint_backward(lam, lam)
+    fb = s.populations()[1]
+    lam = centroid_seed(fb)
+    del fb
+    s.adjoint_backward(lam * 0, lam)
     bwd_ms = s.adjoint_last_ms()
     s.close()
-    return {"workload": "periodic bubble 192x384x192, R=16, tau 0.8, A=1e-3, beta 0.7", "steps": steps,
+    return {"workload": "rising bubble 192x384x192, walls x/z, R=16 at y=64, gravity 1e-5, tau 0.55/1.0, A=1e-3, "
+                        "beta 0.7; centroid-height objective", "steps": steps,
             "checkpoint_every": every, "forward_ms": round(fwd_ms, 3), "backward_ms": round(bwd_ms, 3),
             "ratio": round(bwd_ms / fwd_ms, 3)}
 
