@@ -139,6 +139,49 @@ def make_steps_function():
     return LBMSteps
 
 
+def make_rising_steps_function():
+    """``LBMRisingSteps.apply(fr0, fb0, A, beta, gravity, case, nsteps, checkpoint_every)`` -> (fr, fb):
+    nsteps GPU steps of a periodic-y case (the rising bubble: x/z walls, body
+    force on red, per-node viscosity) from the tanh-bubble initialisation with
+    populations (fr0, fb0) and A_override, beta_recolor, gravity replaced by the
+    tensors A, beta, gravity; backward runs the GPU adjoint and returns
+    gradients for fr0, fb0, A, beta and gravity."""
+    torch = _torch()
+
+    class LBMRisingSteps(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, fr0, fb0, A, beta, gravity, case, nsteps, checkpoint_every):
+            c = case.copy()
+            c.A_override = float(A)
+            c.beta_recolor = float(beta)
+            c.gravity = float(gravity)
+            s = Solver(c)
+            s.init_bubble_tanh()
+            s.set_populations(fr0.detach().cpu().numpy(), fb0.detach().cpu().numpy())
+            s.adjoint_record(nsteps, checkpoint_every)
+            fr, fb = s.populations()
+            ctx.solver = s
+            ctx.device, ctx.dtype = fr0.device, fr0.dtype
+            return torch.from_numpy(fr).to(fr0.device), torch.from_numpy(fb).to(fr0.device)
+
+        @staticmethod
+        def backward(ctx, g_fr, g_fb):
+            s = ctx.solver
+            if g_fr is None:
+                g_fr = torch.zeros((27,) + s.shape)
+            if g_fb is None:
+                g_fb = torch.zeros((27,) + s.shape)
+            lr, lb, _ = s.adjoint_backward(g_fr.detach().cpu().numpy(), g_fb.detach().cpu().numpy())
+            g = s.adjoint_gradients()
+            s.close()
+            ctx.solver = None
+            t = lambda a: torch.from_numpy(a).to(device=ctx.device, dtype=ctx.dtype)  # noqa: E731
+            d64 = lambda v: torch.tensor(v, dtype=torch.float64)  # noqa: E731
+            return t(lr), t(lb), d64(g["dA"]), d64(g["dbeta"]), d64(g["dgravity"]), None, None, None
+
+    return LBMRisingSteps
+
+
 def make_jet_steps_function():
     """``LBMJetSteps.apply(fr0, fb0, A, beta, U, amp, du, case, nsteps, checkpoint_every, w_uy, window)``
     -> (fr, fb, J_uy): nsteps GPU steps of a jet case from the jet initialisation

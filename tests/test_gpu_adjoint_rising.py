@@ -50,3 +50,30 @@ def test_gpu_rising_adjoint_matches_fp64_cpu_adjoint(rising, every):
     assert abs(dbeta - ref["dbeta"]) <= 1e-4 * abs(ref["dbeta"]), (dbeta, ref["dbeta"])
     assert abs(g["dgravity"] - ref["dgravity"]) <= 1e-4 * abs(ref["dgravity"]), (g["dgravity"], ref["dgravity"])
     assert abs(dA - ref["dA"]) <= 2e-3 * abs(ref["dA"]), (dA, ref["dA"])
+
+
+def test_torch_autograd_rising_matches_the_abi(rising):
+    """LBMRisingSteps through torch.autograd: J = <lam, f_N>; gradients equal
+    the direct C-ABI adjoint bit for bit (dJ/dgravity included)."""
+    import torch
+
+    from multicomponent_color_gradient_amd.adjoint import make_rising_steps_function
+
+    case, fr, fb, lam, _ = rising
+    F = make_rising_steps_function()
+    fr0 = torch.tensor(fr, dtype=torch.float32, requires_grad=True)
+    fb0 = torch.tensor(fb, dtype=torch.float32, requires_grad=True)
+    A = torch.tensor(case.A_override, dtype=torch.float64, requires_grad=True)
+    beta = torch.tensor(case.beta_recolor, dtype=torch.float64, requires_grad=True)
+    g = torch.tensor(case.gravity, dtype=torch.float64, requires_grad=True)
+    gr, gb = F.apply(fr0, fb0, A, beta, g, case, STEPS, 2)
+    J = (torch.tensor(lam[0], dtype=torch.float32) * gr).sum() + (torch.tensor(lam[1], dtype=torch.float32) * gb).sum()
+    J.backward()
+    s = m.Solver(case)
+    s.init_bubble_tanh()
+    s.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+    s.adjoint_record(STEPS, 2)
+    lr, lb, _ = s.adjoint_backward(lam[0], lam[1])
+    d = s.adjoint_gradients()
+    assert np.array_equal(fr0.grad.numpy(), lr) and np.array_equal(fb0.grad.numpy(), lb)
+    assert float(A.grad) == d["dA"] and float(beta.grad) == d["dbeta"] and float(g.grad) == d["dgravity"]
