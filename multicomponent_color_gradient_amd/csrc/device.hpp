@@ -393,7 +393,10 @@ __device__ __forceinline__ NodeColl node_setup(const DevParams &P, const Moments
     n.uz = m.uz;
     const float usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
     n.e0 = rT * (1.0f - usq * inv_2cs2);
-    {
+    n.force = P.grav != 0.0f;
+    n.g9 = 0.0f;
+    n.g3 = 0.0f;
+    if (n.force) {
         // body force on red, F_y = -grav rho_r: even source (1 - omega/2)(9 (c.u) c_y - 3 u_y) F_y,
         // odd 1.5 c_y F_y (the regularised collision drops the first-order non-equilibrium
         // moment -F/2, whose (1 - omega) share the odd source restores: sum c f* = j + F)

@@ -238,10 +238,16 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // the collision each appear once in the code, which keeps the loop inside the
 // instruction cache.
 // MOM: the core lanes also store the moments of their node (B.mom_out), the
-// variant the adjoint's recomputation runs.
-template <bool STATS, int ST, bool MOM = false>
+// variant the adjoint's recomputation runs.  EXT: x/z walls or the body force
+// are on; without them the kernel is compiled with both known to be off.
+template <bool STATS, int ST, bool MOM = false, bool EXT = false>
 __global__ __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_per_eu(MCG_FUSED_WAVES_PER_EU)))
 void k_fused(DevParams P, DevBufs B, FusedGrid G) {
+    if constexpr (!EXT) {
+        P.wall = 0;
+        P.grav = 0.0f;
+        P.ghalf = 0.0f;
+    }
     constexpr int TY = kFusedTY;
     constexpr int H = TY + 2;
     constexpr int W = 66;
@@ -431,7 +437,15 @@ template <int ST>
 inline int launch_fused_st(const DevParams &P, const DevBufs &B, hipStream_t stream, int num_cu) {
     const FusedGrid G = fused_grid(P, num_cu);
     const dim3 block(64, kFusedWaves, 1);
-    if (B.do_stats) {
+    if (P.wall || P.grav != 0.0f) {
+        if (B.do_stats) {
+            hipLaunchKernelGGL((k_fused<true, ST, false, true>), dim3(G.nwg), block, fused_stats_lds(P), stream, P, B, G);
+        } else if (B.mom_out) {
+            hipLaunchKernelGGL((k_fused<false, ST, true, true>), dim3(G.nwg), block, 0, stream, P, B, G);
+        } else {
+            hipLaunchKernelGGL((k_fused<false, ST, false, true>), dim3(G.nwg), block, 0, stream, P, B, G);
+        }
+    } else if (B.do_stats) {
         hipLaunchKernelGGL((k_fused<true, ST>), dim3(G.nwg), block, fused_stats_lds(P), stream, P, B, G);
     } else if (B.mom_out) {
         hipLaunchKernelGGL((k_fused<false, ST, true>), dim3(G.nwg), block, 0, stream, P, B, G);
@@ -444,7 +458,10 @@ inline int launch_fused_st(const DevParams &P, const DevBufs &B, hipStream_t str
 // dynamic LDS above the default limit must be allowed per kernel
 template <int ST>
 inline int fused_allow_lds(size_t bytes) {
-    return int(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fused<true, ST>),
+    int rc = int(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fused<true, ST>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    if (rc) return rc;
+    return int(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fused<true, ST, false, true>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
 }
 

@@ -106,7 +106,7 @@ struct MomentAcc {
         const float rho = sr + sb;
         const float inv = 1.0f / rho;
         m.ux = jx * inv;
-        m.uy = (jy - ghalf * sr) * inv;
+        m.uy = (ghalf != 0.0f ? jy - ghalf * sr : jy) * inv;
         m.uz = jz * inv;
         m.pxx = sxx - rho * m.ux * m.ux;
         m.pxy = sxy - rho * m.ux * m.uy;
@@ -168,6 +168,7 @@ struct NodeColl {
     float s2, sB;          // Omega2/w = s2 (c.F)^2 - sB
     float rc;              // D = rc (c.F)/|c|
     float g9, g3;          // body force: E += g9 c_y (c.u), O += g3 c_y (g9 = 9 (1 - omega/2) F_y, g3 = 1.5 F_y)
+    bool force;            // g9, g3 are used only when set (a constant false drops their code)
 };
 
 __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -211,12 +212,14 @@ __device__ __forceinline__ void pair_terms(const NodeColl &n, float &E, float &O
     O = fma_(n.q3, a3, O);
     // body force F = (0, F_y, 0): even part (1 - omega/2)(9 (c.u)(c.F) - 3 u.F) (its
     // u-only term sits in e0), odd part 1.5 c.F
-    if constexpr (kCY[I] > 0) {
-        E = fma_(n.g9, cu, E);
-        O = O + n.g3;
-    } else if constexpr (kCY[I] < 0) {
-        E = fma_(-n.g9, cu, E);
-        O = O - n.g3;
+    if (n.force) {
+        if constexpr (kCY[I] > 0) {
+            E = fma_(n.g9, cu, E);
+            O = O + n.g3;
+        } else if constexpr (kCY[I] < 0) {
+            E = fma_(-n.g9, cu, E);
+            O = O - n.g3;
+        }
     }
     D = n.rc * d;
     if constexpr (csq(I) > 1) D = D * invc(I);
