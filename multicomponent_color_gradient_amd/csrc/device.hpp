@@ -254,14 +254,38 @@ struct Store16 {
     __device__ __forceinline__ static void store(void *f, uint32_t stride, uint32_t off, float fr, float fb) {
         __builtin_nontemporal_store(encode<I>(fr, fb), at(f, uint32_t(I) * stride + off));
     }
-    // word from the offset a = t/w - 1 and the colours already divided by w
-    __device__ __forceinline__ static uint32_t encode_w(float a, float rw, float bw) {
-        const bool red_major = rw >= bw;
-        return pack(a, red_major ? fabsf(bw) : -fabsf(rw));
+    // The same word with stochastic rounding: a uniform random integer below
+    // the dropped mantissa bits (13 for FP16, 16 for BF16) is added to the
+    // FP32 bit pattern, and the result is truncated toward zero.  The pattern
+    // is sign-magnitude and ordered by magnitude, so the magnitude rounds up
+    // with probability equal to the dropped fraction and is truncated
+    // otherwise: E[stored] = value, and the sign is kept (also for -0), so the
+    // colour flag survives.  A steady flow then carries zero-mean storage
+    // noise instead of the same rounding error every step.  FP16: one packed
+    // round-toward-zero convert (v_cvt_pkrtz_f16_f32); below the FP16 normal
+    // range (|x| < 2^-14) more than 13 bits drop and the rounding leans toward
+    // zero by less than 2^-24.  BF16: the two high halves are the word (one
+    // byte permute).  Integer adds are used because the v_cvt_sr_* converts
+    // ran the FP16 step 5% slower.
+    __device__ __forceinline__ static uint32_t pack_sr(float lo, float hi, uint32_t r) {
+        if constexpr (ST == MCG_STORE_FP16) {
+            const uint32_t d = r >> 19;
+            return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(bits_f32(f32_bits(lo) + d),
+                                                                           bits_f32(f32_bits(hi) + d)));
+        } else {
+            const uint32_t d = r >> 16;
+            // bytes 2,3 of (lo + d), then bytes 2,3 of (hi + d)
+            return __builtin_amdgcn_perm(f32_bits(hi) + d, f32_bits(lo) + d, 0x07060302u);
+        }
     }
+    // word from the offset a = t/w - 1 and the colours already divided by w,
+    // stochastically rounded with the random word r (both halves)
     template <int I>
-    __device__ __forceinline__ static void store_w(void *f, uint32_t stride, uint32_t off, float a, float rw, float bw) {
-        __builtin_nontemporal_store(encode_w(a, rw, bw), at(f, uint32_t(I) * stride + off));
+    __device__ __forceinline__ static void store_w(void *f, uint32_t stride, uint32_t off, float a, float rw, float bw,
+                                                   uint32_t r) {
+        const bool red_major = rw >= bw;
+        __builtin_nontemporal_store(pack_sr(a, red_major ? fabsf(bw) : -fabsf(rw), r),
+                                    at(f, uint32_t(I) * stride + off));
     }
 };
 template <>
@@ -456,27 +480,50 @@ __device__ __forceinline__ void collide_node(const NodeColl &n, PutFn &&put) {
     });
 }
 
+// lowbias32, an integer hash with full avalanche (C. Wellons,
+// "Prospecting for hash functions", 2018: xorshift-multiply rounds 16,
+// 0x7feb352d, 15, 0x846ca68b, 16)
+__device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+// Random word of the stochastic rounding of node p's populations at a step:
+// a function of (p, step) only, so every kernel that stores node p at that
+// step draws the same bits and the results stay reproducible.
+__device__ __forceinline__ uint32_t sr_step_key(uint64_t step) { return lowbias32(uint32_t(step) ^ 0x2545F491u); }
+__device__ __forceinline__ uint32_t sr_node_key(uint32_t p, uint32_t step_key) { return lowbias32(p ^ step_key); }
+
 // Collide node n and store its 54 post-collision populations at element p of
 // the row block (FP32 rows, or 16-bit words built directly from the w-scaled
-// terms).
+// terms, stochastically rounded).  Random words: h = sr_node_key(p, step_key)
+// for the rest population, h + k G for the opposite pair (2k-1, 2k), both
+// halves of both words, G = 0x9E3779B9 (a Weyl sequence: distinct high bits
+// for the 14 groups).  Every stored half is unbiased whatever the sharing;
+// sharing only correlates the noise within a node.
 template <int ST>
-__device__ __forceinline__ void collide_store(const NodeColl &n, void *f, uint32_t PX, uint32_t p) {
+__device__ __forceinline__ void collide_store(const NodeColl &n, void *f, uint32_t PX, uint32_t p, uint32_t step_key) {
     if constexpr (ST == MCG_STORE_FP32) {
         collide_node(n, [&](auto Ic, float a, float b) {
             Store<ST>::template store<decltype(Ic)::value>(f, PX, p, a, b);
         });
     } else {
+        const uint32_t h = sr_node_key(p, step_key);
         {
             float E = fma_(n.q2, n.tr, n.e0);
             E = fma_(n.sB, 1.25f, E);
-            Store<ST>::template store_w<0>(f, PX, p, E - 1.0f, n.aR * E, n.aB * E);
+            Store<ST>::template store_w<0>(f, PX, p, E - 1.0f, n.aR * E, n.aB * E, h);
         }
         unroll<0, 13>([&](auto kc) {
             constexpr int I = 2 * decltype(kc)::value + 1;
             float ap, rp, bp, am, rm, bm;
             collide_pair_w<I>(n, ap, rp, bp, am, rm, bm);
-            Store<ST>::template store_w<I>(f, PX, p, ap, rp, bp);
-            Store<ST>::template store_w<I + 1>(f, PX, p, am, rm, bm);
+            const uint32_t r = h + uint32_t(decltype(kc)::value + 1) * 0x9E3779B9u;
+            Store<ST>::template store_w<I>(f, PX, p, ap, rp, bp, r);
+            Store<ST>::template store_w<I + 1>(f, PX, p, am, rm, bm, r);
         });
     }
 }

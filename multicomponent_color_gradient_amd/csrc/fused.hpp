@@ -335,7 +335,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
             const NodeColl n = node_setup(P, m, y, phi);
             const DevParams PL = fresh_strides(P);
             const uint32_t p = pidx(PL, x, y, z);
-            collide_store<ST>(n, B.f_dst, uint32_t(PL.PX), p);
+            collide_store<ST>(n, B.f_dst, uint32_t(PL.PX), p, sr_step_key(B.step));
             if constexpr (MOM) {
                 const uint32_t q = midx(PL, x, y, z);
                 const size_t pl = PL.plane;

@@ -118,3 +118,64 @@ def test_jet_mean_profile_error(mcg, storage, bound):
     prof, mass = mean_profile_errors(mcg, storage)
     assert prof <= bound, f"{storage}: mean-profile error {prof:.3e}"
     assert mass <= 1e-5, f"{storage}: mass error {mass:.3e}"
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_stochastic_rounding_is_unbiased(mcg, storage):
+    """One step of a uniform red flow in a periodic box: every node collides
+    the same populations to the same x_i and stores x_i on the 16-bit grid,
+    rounded up or down with its own random bits.  Per direction the stored
+    values are the two grid neighbours of x_i (FP64 oracle), and their
+    average over 32^3 nodes equals x_i within 5 standard errors of the
+    binomial draw, sigma = g sqrt(p (1 - p) / n) (grid step g, fraction p of
+    the way up).  Round to nearest would sit g min(p, 1 - p) away, which the
+    test tells apart for most directions."""
+    from oracle.oracle import Oracle, feq
+
+    N = 36
+    case = mcg.periodic_bubble_case(N, radius=8.0, tau=0.8)
+    s = mcg.Solver(case, storage=storage)
+    s.init_bubble_tanh()
+    u = (0.021, 0.047, -0.033)
+    fr = np.empty((27,) + s.shape, np.float32)
+    for i in range(27):
+        fr[i] = feq(i, 1.0, *u)
+    s.set_populations(fr, np.zeros_like(fr))
+    r0, b0 = s.populations()
+    o = Oracle(case)
+    o.init_bubble_tanh()
+    o.set_populations(r0, b0)
+    o.step(1)
+    s.step(1)
+    xr, xb = o.populations()
+    gr, gb = s.populations()
+    core = (slice(None),) + (slice(2, -2),) * 3
+    t = (gr + gb)[core].reshape(27, -1).astype(np.float64)
+    x = (xr + xb)[core].reshape(27, -1)
+    n = t.shape[1]
+    assert np.all(gb[core] == 0)
+    assert np.ptp(x, axis=1).max() <= 1e-15
+    zs, powered = [], 0
+    for i in range(27):
+        xi = float(x[i, 0])
+        vals = np.unique(t[i])
+        assert len(vals) <= 2, (i, vals)
+        lo, hi = float(vals[0]), float(vals[-1])
+        # the GPU collides in FP32: its x_i is within 2.1e-7 relative of the
+        # FP64 value (the oracle's FP32 mode on this state); allow 5e-7
+        tol = 5e-7 * abs(xi)
+        assert lo - tol <= xi <= hi + tol, (i, xi, vals)
+        if len(vals) == 1:
+            assert abs(lo - xi) <= tol, (i, xi, lo)
+            continue
+        g = hi - lo
+        p = min(max((xi - lo) / g, 0.0), 1.0)
+        sigma = g * np.sqrt(max(p * (1 - p), 1.0 / n) / n)
+        z = abs(t[i].mean() - xi) / (sigma + tol)
+        zs.append(z)
+        assert z <= 5.0, f"{storage} direction {i}: mean off by {z:.1f} sigma (p={p:.3f})"
+        if g * min(p, 1 - p) > 5 * (sigma + tol):
+            powered += 1
+    print(f"{storage}: max |mean - x| = {max(zs):.2f} sigma over {len(zs)} directions; "
+          f"round to nearest would be >5 sigma off in {powered}", flush=True)
+    assert powered >= 10
