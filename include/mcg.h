@@ -55,6 +55,9 @@ extern "C" {
 #define MCG_STORE_FP32 0
 #define MCG_STORE_FP16 1 /* fp16 offsets from the lattice weights, fp32 arithmetic */
 #define MCG_STORE_BF16 2 /* bf16 offsets from the lattice weights, fp32 arithmetic */
+/* With 16-bit storage the steps store every population with stochastic
+ * rounding: unbiased, with random bits from a hash of (node, step), so a run
+ * is reproducible bit for bit.  mcg_set_populations rounds to nearest. */
 
 /* step kernel variant */
 #define MCG_KERNEL_FUSED 0   /* one fused launch per step (default)                    */
