@@ -258,6 +258,22 @@ int mcg_checkpoint_load(mcg_solver *s, const void *host_buf, uint64_t nbytes);
  * follow mcg_init_jet. */
 int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every);
 
+/* Cap the states mcg_adjoint_record keeps (max_states >= 1; 0, the default,
+ * keeps one per segment of K steps).  Below one per segment the record keeps
+ * the first state and the split points of the sweep's first descent, and
+ * mcg_adjoint_backward recomputes the missing segment states with a binomial
+ * schedule (Griewank and Walther's revolve): memory max_states + 2 state sets
+ * and K moment sets for any nsteps, at the cost of recomputed forward steps
+ * (mcg_adjoint_recomputed_steps).  The gradients are bitwise those of the
+ * unlimited record.  Applies to the next mcg_adjoint_record.  No counterpart
+ * in the reference (an extension). */
+int mcg_adjoint_set_checkpoint_limit(mcg_solver *s, int max_states);
+
+/* Forward steps recomputed by the last mcg_adjoint_backward: nsteps with one
+ * state per segment, more under mcg_adjoint_set_checkpoint_limit.  No
+ * counterpart in the reference (an extension). */
+int mcg_adjoint_recomputed_steps(const mcg_solver *s, int64_t *nsteps);
+
 /* Jet cases: define the u_y functional J_uy = sum over the moments passes of
  * steps first..last (solver step numbers: the pass of the step that takes
  * step n to n+1 is pass n) of sum_x w_uy(x) u_y(x) (w_uy: NX*NY*NZ floats, x

@@ -143,6 +143,8 @@ SIGNATURES = [
     ("mcg_adjoint_backward", c_int, [_PSOLVER, _F, _F, _F, _F, _D]),
     ("mcg_adjoint_last_ms", c_int, [_PSOLVER, _D]),
     ("mcg_adjoint_gradients", c_int, [_PSOLVER, _D, c_int]),
+    ("mcg_adjoint_set_checkpoint_limit", c_int, [_PSOLVER, c_int]),
+    ("mcg_adjoint_recomputed_steps", c_int, [_PSOLVER, POINTER(c_int64)]),
     ("mcg_adjoint_release", c_int, [_PSOLVER]),
     ("mcg_adjoint_set_uy_seed", c_int, [_PSOLVER, _F, c_int64, c_int64]),
     ("mcg_adjoint_inlet_gradient", c_int, [_PSOLVER, _D, _D]),

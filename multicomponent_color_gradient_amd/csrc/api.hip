@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/mcg.h"
@@ -100,7 +101,14 @@ struct mcg_solver {
     bool adj_recorded = false;
     int adj_nsteps = 0, adj_every = 0;
     int64_t adj_step0 = 0;
+    // recorded states: adj_ckpt holds the state buffers ("slots"); adj_seg_slot
+    // maps segment k (steps k*every ..) to the slot holding its first state, or
+    // -1.  adj_limit caps the slots (0: one per segment); below one per segment
+    // the sweep recomputes the missing states (binomial schedule).
     std::vector<void *> adj_ckpt;
+    std::vector<int> adj_seg_slot;
+    int adj_limit = 0;
+    int64_t adj_recomputed = 0;  // forward steps recomputed by the last backward sweep
     void *adj_pp[2] = {nullptr, nullptr};  // alternating states of the segment recomputation
     float *adj_mom = nullptr;  // moments of the states of one segment
     float *adj_gbs = nullptr;  // jet cases: outlet states of the states of one segment
@@ -121,6 +129,7 @@ struct mcg_solver {
     void adj_release() {
         for (void *p : adj_ckpt) (void)hipFree(p);
         adj_ckpt.clear();
+        adj_seg_slot.clear();
         for (void *p : {adj_pp[0], adj_pp[1], (void *)adj_mom, (void *)adj_gbs, (void *)adj_lam[0], (void *)adj_lam[1], (void *)adj_lm, (void *)adj_lF,
                         (void *)adj_ps, (void *)adj_acc, (void *)adj_part, (void *)adj_lg[0], (void *)adj_lg[1],
                         (void *)adj_din, (void *)adj_psf})
@@ -923,6 +932,28 @@ static int adj_supported(const mcg_case &c, int storage) {
     return MCG_OK;
 }
 
+// Binomial checkpointing (Griewank and Walther, "Algorithm 799: revolve",
+// ACM TOMS 26(1), 2000).  With f free state slots and t repetitions at most
+// beta(f, t) = C(f + t, f) segments can be reversed: the first f slots split
+// the range so that the right part needs f - 1 slots and t repetitions and the
+// left part f slots and t - 1.  Returns the split point m of [a, b) for f >= 1
+// free slots: the smallest t with beta(f, t) >= b - a, then the right part as
+// large as beta(f - 1, t) allows.
+static double revolve_beta(int f, int t) {
+    double v = 1.0;
+    for (int i = 1; i <= f; ++i) v = v * double(t + i) / double(i);
+    return v;
+}
+
+static int revolve_split(int a, int b, int f) {
+    const int L = b - a;
+    int t = 0;
+    while (revolve_beta(f, t) < double(L)) ++t;
+    const double right = revolve_beta(f - 1, t);
+    const int left = std::max(1, L - int(std::min(right, double(L))));
+    return a + std::min(left, L - 1);
+}
+
 // copy the solver's current state (populations and, for jet cases, the
 // boundary arrays) into a recorded state buffer
 static int save_state(mcg_solver *s, void *st) {
@@ -944,8 +975,9 @@ int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every) {
     if (s->inlet_reset_pending && s->init_kind != kInitJet)
         return fail(MCG_E_UNSUPPORTED, "the adjoint of the first jet step needs the jet initialisation (take one step first)");
     s->adj_release();
-    const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
-    for (int k = 0; k < nck; ++k) {
+    const int nseg = (nsteps + checkpoint_every - 1) / checkpoint_every;
+    const int nslot = s->adj_limit > 0 ? std::min(nseg, s->adj_limit) : nseg;
+    for (int k = 0; k < nslot; ++k) {
         void *p = nullptr;
         if (hipMalloc(&p, s->state_bytes()) != hipSuccess) {
             s->adj_release();
@@ -953,10 +985,26 @@ int mcg_adjoint_record(mcg_solver *s, int nsteps, int checkpoint_every) {
         }
         s->adj_ckpt.push_back(p);
     }
+    // the states stored now: every segment, or the first descent of the
+    // sweep's schedule (segment 0, then each split point toward the end)
+    s->adj_seg_slot.assign(size_t(nseg), -1);
+    if (nslot == nseg) {
+        for (int k = 0; k < nseg; ++k) s->adj_seg_slot[size_t(k)] = k;
+    } else {
+        int used = 0, a = 0;
+        s->adj_seg_slot[0] = used++;
+        while (used < nslot && nseg - a > 1) {
+            a = revolve_split(a, nseg, nslot - used);
+            s->adj_seg_slot[size_t(a)] = used++;
+        }
+    }
     s->adj_step0 = s->step;
     for (int k = 0; k < nsteps; ++k) {
-        if (k % checkpoint_every == 0)
-            if (int rc = save_state(s, s->adj_ckpt[k / checkpoint_every])) return rc;
+        if (k % checkpoint_every == 0) {
+            const int slot = s->adj_seg_slot[size_t(k / checkpoint_every)];
+            if (slot >= 0)
+                if (int rc = save_state(s, s->adj_ckpt[size_t(slot)])) return rc;
+        }
         const int rc = one_step(s);
         if (rc) return rc;
     }
@@ -1182,37 +1230,116 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
     HIP_TRY(hipEventRecord(e0, s->stream));
     int a = 0;
     int rc = MCG_OK;
-    const int nck = int(s->adj_ckpt.size());
+    const int nseg = int(s->adj_seg_slot.size()), K = s->adj_every;
     const size_t mset = s->mom_set_bytes() / sizeof(float), gset = s->gB_set_bytes() / sizeof(float);
     const bool jet = s->jet();
-    for (int seg = nck - 1; seg >= 0 && rc == MCG_OK; --seg) {
-        const int n0 = seg * s->adj_every, n1 = std::min(n0 + s->adj_every, s->adj_nsteps);
-        // Recompute the segment from its checkpoint through two alternating
-        // state buffers; each step records the moments (and, jet cases, the
-        // outlet state) of its source state, all the adjoint reads.  The last
-        // step's output goes to the solver's other buffer.
+    s->adj_recomputed = 0;
+    auto state = [&](void *p) { return StateRef{p, s->st_bin(p), s->st_bout(p), s->st_gB(p)}; };
+    // Reverse segment seg from its first state in buffer sb: recompute the
+    // segment through the two alternating buffers (the one that is not sb
+    // first), each step recording the moments (and, jet cases, the outlet
+    // state) of its source state, all the adjoint reads; the last step's
+    // output goes to the solver's other buffer.  Then the adjoint steps.
+    auto reverse_segment = [&](int seg, void *sb) -> int {
+        const int n0 = seg * K, n1 = std::min(n0 + K, s->adj_nsteps);
         auto moms = [&](int k) -> float * { return s->adj_mom + size_t(k - n0) * mset; };
         auto gbs = [&](int k) -> float * { return jet ? s->adj_gbs + size_t(k - n0) * gset : nullptr; };
-        void *ck = s->adj_ckpt[seg];
-        StateRef src{ck, s->st_bin(ck), s->st_bout(ck), gbs(n0)};
-        if (jet) HIP_TRY(hipMemcpyAsync(gbs(n0), s->st_gB(ck), s->gB_bytes(), hipMemcpyDeviceToDevice, s->stream));
+        StateRef src{sb, s->st_bin(sb), s->st_bout(sb), gbs(n0)};
+        if (jet) HIP_TRY(hipMemcpyAsync(gbs(n0), s->st_gB(sb), s->gB_bytes(), hipMemcpyDeviceToDevice, s->stream));
         s->step = s->adj_step0 + n0;
-        for (int k = n0; k < n1 && rc == MCG_OK; ++k) {
+        int x = sb == s->adj_pp[0] ? 1 : 0;
+        for (int k = n0; k < n1; ++k) {
             StateRef dst;
             if (k + 1 < n1) {
-                void *p = s->adj_pp[(k - n0) & 1];
+                void *p = s->adj_pp[x];
+                x ^= 1;
                 dst = StateRef{p, s->st_bin(p), s->st_bout(p), gbs(k + 1)};
             } else {
                 dst = StateRef{s->f[cur0 ^ 1], s->bin[cur0 ^ 1], s->bout[cur0 ^ 1], s->gB[cur0 ^ 1]};
             }
-            rc = step_from(s, src, dst, moms(k));
+            if (int r = step_from(s, src, dst, moms(k))) return r;
             src = dst;
         }
-        for (int k = n1 - 1; k >= n0 && rc == MCG_OK; --k) {
-            rc = adjoint_step(s, k, gbs(k), moms(k), s->adj_lam[a], s->adj_lam[a ^ 1], s->adj_lg[a], s->adj_lg[a ^ 1]);
+        s->adj_recomputed += n1 - n0;
+        for (int k = n1 - 1; k >= n0; --k) {
+            if (int r = adjoint_step(s, k, gbs(k), moms(k), s->adj_lam[a], s->adj_lam[a ^ 1], s->adj_lg[a], s->adj_lg[a ^ 1]))
+                return r;
             a ^= 1;
         }
+        return MCG_OK;
+    };
+    // Advance from the first state of segment sa (buffer sb) to the first
+    // state of segment sm, into buffer to (null: whichever alternating
+    // buffer the last step writes, returned in *out).
+    auto advance = [&](int sa, void *sb, int sm, void *to, void **out) -> int {
+        StateRef src = state(sb);
+        int x = 0;
+        const int n_end = sm * K;
+        for (int n = sa * K; n < n_end; ++n) {
+            void *p = (to && n + 1 == n_end) ? to : s->adj_pp[x];
+            x ^= 1;
+            const StateRef dst = state(p);
+            s->step = s->adj_step0 + n;
+            if (int r = step_from(s, src, dst, nullptr)) return r;
+            src = dst;
+        }
+        s->adj_recomputed += n_end - sa * K;
+        if (out) *out = src.f;
+        return MCG_OK;
+    };
+    std::vector<int> free_slots;
+    std::vector<char> rewritten(s->adj_ckpt.size(), 0);  // slots whose recorded state the sweep replaced
+    {
+        std::vector<char> used(s->adj_ckpt.size(), 0);
+        for (int v : s->adj_seg_slot)
+            if (v >= 0) used[size_t(v)] = 1;
+        for (size_t v = 0; v < used.size(); ++v)
+            if (!used[v]) {
+                free_slots.push_back(int(v));
+                if (jet) {  // stale inlet/outlet entries, as in every recorded state
+                    void *p = s->adj_ckpt[v];
+                    HIP_TRY(hipMemcpyAsync(s->st_bin(p), s->st_bin(s->adj_ckpt[0]), s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+                    HIP_TRY(hipMemcpyAsync(s->st_bout(p), s->st_bout(s->adj_ckpt[0]), s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
+                }
+            }
     }
+    // Reverse segments [sa, sb): the first state of sa is stored.  Slots for
+    // this range: the free ones and those holding stored states inside it
+    // (the record's first descent), so the splits repeat the record's.
+    std::function<int(int, int)> rev = [&](int sa, int sb) -> int {
+        while (sb - sa > 1) {
+            int f = int(free_slots.size());
+            for (int j = sa + 1; j < sb; ++j) f += s->adj_seg_slot[size_t(j)] >= 0;
+            void *base = s->adj_ckpt[size_t(s->adj_seg_slot[size_t(sa)])];
+            if (f == 0) {  // no slot: recompute every segment's first state from sa
+                for (int j = sb - 1; j > sa; --j) {
+                    void *p = nullptr;
+                    if (int r = advance(sa, base, j, nullptr, &p)) return r;
+                    if (int r = reverse_segment(j, p)) return r;
+                }
+                break;
+            }
+            const int m = revolve_split(sa, sb, f);
+            if (s->adj_seg_slot[size_t(m)] < 0) {
+                const int slot = free_slots.back();
+                free_slots.pop_back();
+                if (int r = advance(sa, base, m, s->adj_ckpt[size_t(slot)], nullptr)) return r;
+                s->adj_seg_slot[size_t(m)] = slot;
+                rewritten[size_t(slot)] = 1;
+            }
+            if (int r = rev(m, sb)) return r;
+            free_slots.push_back(s->adj_seg_slot[size_t(m)]);
+            s->adj_seg_slot[size_t(m)] = -1;
+            sb = m;
+        }
+        return reverse_segment(sa, s->adj_ckpt[size_t(s->adj_seg_slot[size_t(sa)])]);
+    };
+    const std::vector<int> seg_slot0 = s->adj_seg_slot;
+    rc = rev(0, nseg);
+    // recorded states the sweep did not overwrite stay usable for another sweep
+    s->adj_seg_slot = seg_slot0;
+    for (int &v : s->adj_seg_slot)
+        if (v >= 0 && rewritten[size_t(v)]) v = -1;
     s->stats_frozen = false;
     (void)hipEventRecord(e1, s->stream);
     (void)hipEventSynchronize(e1);
@@ -1240,6 +1367,20 @@ int mcg_adjoint_backward(mcg_solver *s, const float *lam_fr, const float *lam_fb
         dparams[0] = acc[0];
         dparams[1] = acc[1];
     }
+    return MCG_OK;
+}
+
+int mcg_adjoint_set_checkpoint_limit(mcg_solver *s, int max_states) {
+    if (int rc = check_solver(s)) return rc;
+    if (max_states < 0) return fail(MCG_E_INVALID, "max_states must be >= 0 (0: one state per segment)");
+    s->adj_limit = max_states;
+    return MCG_OK;
+}
+
+int mcg_adjoint_recomputed_steps(const mcg_solver *s, int64_t *nsteps) {
+    if (int rc = check_solver(s)) return rc;
+    if (!nsteps) return fail(MCG_E_INVALID, "null pointer");
+    *nsteps = s->adj_recomputed;
     return MCG_OK;
 }
 

@@ -178,6 +178,17 @@ class Solver:
         w = np.ascontiguousarray(w_uy, dtype=np.float32).reshape(self.shape)
         check(self.lib.mcg_adjoint_set_uy_seed(self._h, as_ptr(w, ctypes.c_float), int(first), int(last)))
 
+    def adjoint_set_checkpoint_limit(self, max_states: int) -> None:
+        """Cap the states the next adjoint_record keeps (0: one per segment); the sweep then
+        recomputes the missing ones with a binomial (revolve) schedule, same gradients."""
+        check(self.lib.mcg_adjoint_set_checkpoint_limit(self._h, int(max_states)))
+
+    def adjoint_recomputed_steps(self) -> int:
+        """Forward steps recomputed by the last adjoint_backward."""
+        v = ctypes.c_int64()
+        check(self.lib.mcg_adjoint_recomputed_steps(self._h, ctypes.byref(v)))
+        return v.value
+
     def adjoint_gradients(self) -> dict:
         """FP64 parameter gradients of the last adjoint_backward sweep."""
         out = np.zeros(5, np.float64)
