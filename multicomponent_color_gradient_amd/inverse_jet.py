@@ -92,11 +92,14 @@ def functional(case: McgCase, steps: int, last: int, y0: int, y1: int, du=None) 
     return J
 
 
-def gradient(case: McgCase, steps: int, last: int, y0: int, y1: int, checkpoint_every: int = 200, du=None) -> dict:
+def gradient(case: McgCase, steps: int, last: int, y0: int, y1: int, checkpoint_every: int = 200, du=None,
+             max_states: int = 0) -> dict:
     """J and dJ/d(du) per column (3, NZ, NX) by the adjoint, plus dJ/djet_velocity,
-    dJ/dperturbation_amp and the device times of the forward and backward sweeps."""
+    dJ/dperturbation_amp and the device times of the forward and backward sweeps.
+    max_states > 0 caps the stored states (binomial recomputation schedule)."""
     w = centreline_weights(case, y0, y1, last)
     s = _solver(case, du, w, (steps - last, steps - 1))
+    s.adjoint_set_checkpoint_limit(max_states)
     t0 = time.perf_counter()
     s.adjoint_record(steps, checkpoint_every)
     record_s = time.perf_counter() - t0
@@ -104,9 +107,11 @@ def gradient(case: McgCase, steps: int, last: int, y0: int, y1: int, checkpoint_
     zero = np.zeros((27, case.NZ, case.NY, case.NX), np.float32)
     s.adjoint_backward(zero, zero)
     bwd_ms = s.adjoint_last_ms()
+    recomputed = s.adjoint_recomputed_steps()
     (dU, damp), din = s.adjoint_inlet_gradient()
     s.close()
-    return {"J": J, "dinlet": din, "dU": dU, "damp": damp, "record_s": record_s, "backward_ms": bwd_ms}
+    return {"J": J, "dinlet": din, "dU": dU, "damp": damp, "record_s": record_s, "backward_ms": bwd_ms,
+            "recomputed_steps": recomputed}
 
 
 def main(argv=None) -> int:
@@ -120,17 +125,20 @@ def main(argv=None) -> int:
     ap.add_argument("--y0", type=int, default=150)
     ap.add_argument("--y1", type=int, default=200)
     ap.add_argument("--every", type=int, default=200, help="checkpoint interval K")
+    ap.add_argument("--max-states", type=int, default=0,
+                    help="cap on the stored states (0: one per segment); the sweep recomputes the rest")
     ap.add_argument("--directions", type=int, default=8)
     ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args(argv)
     case = inverse_jet_case(args.NX, args.NY, args.NZ, args.radius)
-    g = gradient(case, args.steps, args.last, args.y0, args.y1, args.every)
+    g = gradient(case, args.steps, args.last, args.y0, args.y1, args.every, max_states=args.max_states)
     dirs = random_directions(case, args.directions, args.seed)
     out = {"case": f"perturbed jet {case.NX}x{case.NY}x{case.NZ}, R={case.jet_radius:g}", "steps": args.steps,
            "window": [args.steps - args.last, args.steps - 1], "y": [args.y0, args.y1],
            "jet_columns": int(jet_mask(case).sum()), "J": g["J"], "grad_norm": float(np.linalg.norm(g["dinlet"])),
            "dJ_dU": g["dU"], "dJ_damp": g["damp"],
            "projections": [float(np.sum(g["dinlet"] * d)) for d in dirs],
+           "checkpoint_every": args.every, "max_states": args.max_states, "recomputed_steps": g["recomputed_steps"],
            "record_s": round(g["record_s"], 3), "backward_ms": round(g["backward_ms"], 3)}
     print(json.dumps(out))
     return 0

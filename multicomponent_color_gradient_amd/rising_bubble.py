@@ -56,10 +56,12 @@ def forward_J(case: McgCase, steps: int) -> float:
     return J
 
 
-def gradient(case: McgCase, steps: int, every: int) -> dict:
-    """J, dJ/dA, dJ/dbeta, dJ/dgravity by the adjoint, and the device times."""
+def gradient(case: McgCase, steps: int, every: int, max_states: int = 0) -> dict:
+    """J, dJ/dA, dJ/dbeta, dJ/dgravity by the adjoint, and the device times.
+    max_states > 0 caps the stored states (binomial recomputation schedule)."""
     s = Solver(case)
     s.init_bubble_tanh()
+    s.adjoint_set_checkpoint_limit(max_states)
     t0 = time.perf_counter()
     s.adjoint_record(steps, every)
     record_s = time.perf_counter() - t0
@@ -70,7 +72,7 @@ def gradient(case: McgCase, steps: int, every: int) -> dict:
     s.adjoint_backward(np.zeros_like(lam_b), lam_b)
     g = s.adjoint_gradients()
     out = {"J": J, "dA": g["dA"], "dbeta": g["dbeta"], "dgravity": g["dgravity"], "record_s": record_s,
-           "backward_ms": s.adjoint_last_ms()}
+           "backward_ms": s.adjoint_last_ms(), "recomputed_steps": s.adjoint_recomputed_steps()}
     s.close()
     return out
 
@@ -83,12 +85,15 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--every", type=int, default=200, help="checkpoint interval K")
     ap.add_argument("--fd-rel", type=float, default=1e-3, help="relative FD step (0: no FD)")
+    ap.add_argument("--max-states", type=int, default=0,
+                    help="cap on the stored states (0: one per segment); the sweep recomputes the rest")
     args = ap.parse_args(argv)
     case = rising_bubble_case(args.NX, args.NY, args.NZ)
-    g = gradient(case, args.steps, args.every)
+    g = gradient(case, args.steps, args.every, args.max_states)
     out = {"case": f"rising bubble {case.NX}x{case.NY}x{case.NZ}, R={case.bubble_radius:g} at y={case.bubble_y0:g}, "
                    f"gravity {case.gravity:g}, tau {case.taur:g}/{case.taub:g}, A={case.A_override:g}",
            "steps": args.steps, "checkpoint_every": args.every, **{k: g[k] for k in ("J", "dA", "dgravity", "dbeta")},
+           "max_states": args.max_states, "recomputed_steps": g["recomputed_steps"],
            "record_s": round(g["record_s"], 3), "backward_ms": round(g["backward_ms"], 3)}
     if args.fd_rel > 0:
         for name, key in (("A_override", "dA"), ("gravity", "dgravity")):
