@@ -101,21 +101,24 @@ def _torch():
 
 
 def make_steps_function():
-    """``LBMSteps.apply(fr0, fb0, A, beta, case, nsteps, checkpoint_every)`` -> (fr, fb):
+    """``LBMSteps.apply(fr0, fb0, A, beta, case, nsteps, checkpoint_every[, max_states])`` -> (fr, fb):
     nsteps GPU steps from (fr0, fb0) with the case's A_override / beta_recolor
     replaced by the tensors A and beta; backward runs the GPU adjoint with
-    checkpointed recomputation."""
+    checkpointed recomputation (max_states > 0: at most that many stored
+    states, binomial recomputation, the same gradients)."""
     torch = _torch()
 
     class LBMSteps(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, fr0, fb0, A, beta, case, nsteps, checkpoint_every):
+        def forward(ctx, fr0, fb0, A, beta, case, nsteps, checkpoint_every, *limit):
             c = case.copy()
             c.A_override = float(A)
             c.beta_recolor = float(beta)
             s = Solver(c)
             s.init_bubble_tanh()
             s.set_populations(fr0.detach().cpu().numpy(), fb0.detach().cpu().numpy())
+            s.adjoint_set_checkpoint_limit(int(limit[0]) if limit else 0)
+            ctx.n_limit = len(limit)
             s.adjoint_record(nsteps, checkpoint_every)
             fr, fb = s.populations()
             ctx.solver = s
@@ -134,13 +137,13 @@ def make_steps_function():
             ctx.solver = None
             t = lambda a: torch.from_numpy(a).to(device=ctx.device, dtype=ctx.dtype)  # noqa: E731
             return (t(lr), t(lb), torch.tensor(dA, dtype=torch.float64), torch.tensor(dbeta, dtype=torch.float64),
-                    None, None, None)
+                    None, None, None) + (None,) * ctx.n_limit
 
     return LBMSteps
 
 
 def make_rising_steps_function():
-    """``LBMRisingSteps.apply(fr0, fb0, A, beta, gravity, case, nsteps, checkpoint_every)`` -> (fr, fb):
+    """``LBMRisingSteps.apply(fr0, fb0, A, beta, gravity, case, nsteps, checkpoint_every[, max_states])`` -> (fr, fb):
     nsteps GPU steps of a periodic-y case (the rising bubble: x/z walls, body
     force on red, per-node viscosity) from the tanh-bubble initialisation with
     populations (fr0, fb0) and A_override, beta_recolor, gravity replaced by the
@@ -150,7 +153,7 @@ def make_rising_steps_function():
 
     class LBMRisingSteps(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, fr0, fb0, A, beta, gravity, case, nsteps, checkpoint_every):
+        def forward(ctx, fr0, fb0, A, beta, gravity, case, nsteps, checkpoint_every, *limit):
             c = case.copy()
             c.A_override = float(A)
             c.beta_recolor = float(beta)
@@ -158,6 +161,8 @@ def make_rising_steps_function():
             s = Solver(c)
             s.init_bubble_tanh()
             s.set_populations(fr0.detach().cpu().numpy(), fb0.detach().cpu().numpy())
+            s.adjoint_set_checkpoint_limit(int(limit[0]) if limit else 0)
+            ctx.n_limit = len(limit)
             s.adjoint_record(nsteps, checkpoint_every)
             fr, fb = s.populations()
             ctx.solver = s
@@ -177,13 +182,13 @@ def make_rising_steps_function():
             ctx.solver = None
             t = lambda a: torch.from_numpy(a).to(device=ctx.device, dtype=ctx.dtype)  # noqa: E731
             d64 = lambda v: torch.tensor(v, dtype=torch.float64)  # noqa: E731
-            return t(lr), t(lb), d64(g["dA"]), d64(g["dbeta"]), d64(g["dgravity"]), None, None, None
+            return (t(lr), t(lb), d64(g["dA"]), d64(g["dbeta"]), d64(g["dgravity"]), None, None, None) + (None,) * ctx.n_limit
 
     return LBMRisingSteps
 
 
 def make_jet_steps_function():
-    """``LBMJetSteps.apply(fr0, fb0, A, beta, U, amp, du, case, nsteps, checkpoint_every, w_uy, window)``
+    """``LBMJetSteps.apply(fr0, fb0, A, beta, U, amp, du, case, nsteps, checkpoint_every, w_uy, window[, max_states])``
     -> (fr, fb, J_uy): nsteps GPU steps of a jet case from the jet initialisation
     with populations (fr0, fb0), A_override, beta_recolor, jet_velocity and
     perturbation_amp replaced by the tensors A, beta, U, amp, and the inlet
@@ -195,7 +200,7 @@ def make_jet_steps_function():
 
     class LBMJetSteps(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, fr0, fb0, A, beta, U, amp, du, case, nsteps, checkpoint_every, w_uy, window):
+        def forward(ctx, fr0, fb0, A, beta, U, amp, du, case, nsteps, checkpoint_every, w_uy, window, *limit):
             c = case.copy()
             c.A_override = float(A)
             c.beta_recolor = float(beta)
@@ -208,6 +213,8 @@ def make_jet_steps_function():
             w = None if w_uy is None else np.ascontiguousarray(w_uy.detach().cpu().numpy(), np.float32)
             if w is not None:
                 s.adjoint_set_uy_seed(w, int(window[0]), int(window[1]))
+            s.adjoint_set_checkpoint_limit(int(limit[0]) if limit else 0)
+            ctx.n_limit = len(limit)
             s.adjoint_record(nsteps, checkpoint_every)
             J = s.uy_functional() if w is not None else 0.0
             fr, fb = s.populations()
@@ -233,7 +240,7 @@ def make_jet_steps_function():
             t = lambda a: torch.from_numpy(a).to(device=ctx.device, dtype=ctx.dtype)  # noqa: E731
             d64 = lambda v: torch.tensor(v, dtype=torch.float64)  # noqa: E731
             return (t(lr), t(lb), d64(dA), d64(dbeta), d64(dU), d64(damp), torch.from_numpy(din),
-                    None, None, None, None, None)
+                    None, None, None, None, None) + (None,) * ctx.n_limit
 
     return LBMJetSteps
 

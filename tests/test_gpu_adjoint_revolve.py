@@ -148,3 +148,43 @@ def test_jet_gradients_independent_of_the_checkpoint_limit():
         same(res[limit][0], res[0][0])
         assert res[limit][1:3] == res[0][1:3]
         assert np.array_equal(res[limit][3], res[0][3])
+
+
+def test_torch_jet_steps_with_a_state_cap():
+    """LBMJetSteps with max_states = 2 (trailing argument): the gradients equal
+    those of the unlimited record through the C ABI, bit for bit."""
+    import torch
+
+    from multicomponent_color_gradient_amd.adjoint import make_jet_steps_function
+
+    case = base_case()
+    fr, fb, _ = initial_state(case)
+    w = weights(case)
+    rng = np.random.default_rng(7)
+    lam = [rng.standard_normal(fr.shape).astype(np.float32) * 0.1 for _ in range(2)]
+    du = np.zeros((3, case.NZ, case.NX), np.float32)
+    print(f"torch jet {case.NX}x{case.NY}x{case.NZ}: {JET_STEPS} steps, every 1, max_states 2", flush=True)
+    F = make_jet_steps_function()
+    fr0 = torch.tensor(fr, dtype=torch.float32, requires_grad=True)
+    fb0 = torch.tensor(fb, dtype=torch.float32, requires_grad=True)
+    A = torch.tensor(case.A_override, dtype=torch.float64, requires_grad=True)
+    beta = torch.tensor(case.beta_recolor, dtype=torch.float64, requires_grad=True)
+    U = torch.tensor(case.jet_velocity, dtype=torch.float64, requires_grad=True)
+    amp = torch.tensor(case.perturbation_amp, dtype=torch.float64, requires_grad=True)
+    dut = torch.tensor(du, dtype=torch.float64, requires_grad=True)
+    gr, gb, juy = F.apply(fr0, fb0, A, beta, U, amp, dut, case, JET_STEPS, 1, torch.tensor(w), UY_STEPS, 2)
+    J = (torch.tensor(lam[0]) * gr).sum() + (torch.tensor(lam[1]) * gb).sum()
+    (J.double() + juy).backward()
+    s = m.Solver(case)
+    s.init_jet()
+    s.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+    s.set_inlet_offset(du)
+    s.adjoint_set_uy_seed(w, *UY_STEPS)
+    s.adjoint_record(JET_STEPS, 1)
+    lr, lb, (dA, dbeta) = s.adjoint_backward(lam[0], lam[1])
+    (dU, damp), din = s.adjoint_inlet_gradient()
+    s.close()
+    assert np.array_equal(fr0.grad.numpy(), lr) and np.array_equal(fb0.grad.numpy(), lb)
+    assert float(A.grad) == dA and float(beta.grad) == dbeta
+    assert float(U.grad) == dU and float(amp.grad) == damp
+    assert np.array_equal(dut.grad.numpy(), din)
