@@ -17,7 +17,13 @@ returns dJ/d(du) per column in FP64 (``adjoint_inlet_gradient``).
 Command line (one GPU):
   python -m multicomponent_color_gradient_amd.inverse_jet [--steps 2000 --last 500 --y0 150 --y1 200]
 prints one JSON line: J, the gradient's norm and its projections on seeded
-random directions, and the forward / backward device times.
+random directions, and the forward / backward device times.  ``--fd-h h``
+adds central differences (J(h d) - J(-h d)) / 2h of the FP32 forward along
+each direction d, and their relative differences from the projections.  Over
+2000 steps the FP32 forward is not smooth enough for these to check the
+adjoint closer than a few per cent (they differ by up to 5% between h = 1e-3
+and 1e-2); central differences of the FP64 oracle with h = 1e-4 agree with the
+adjoint to 2e-5.
 """
 from __future__ import annotations
 
@@ -128,6 +134,8 @@ def main(argv=None) -> int:
     ap.add_argument("--max-states", type=int, default=0,
                     help="cap on the stored states (0: one per segment); the sweep recomputes the rest")
     ap.add_argument("--directions", type=int, default=8)
+    ap.add_argument("--fd-h", type=float, default=0.0,
+                    help="central-difference step along each direction (0: no finite differences)")
     ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args(argv)
     case = inverse_jet_case(args.NX, args.NY, args.NZ, args.radius)
@@ -140,6 +148,18 @@ def main(argv=None) -> int:
            "projections": [float(np.sum(g["dinlet"] * d)) for d in dirs],
            "checkpoint_every": args.every, "max_states": args.max_states, "recomputed_steps": g["recomputed_steps"],
            "record_s": round(g["record_s"], 3), "backward_ms": round(g["backward_ms"], 3)}
+    if args.fd_h > 0:
+        h = args.fd_h
+        fd = []
+        for d in dirs:
+            jp = functional(case, args.steps, args.last, args.y0, args.y1, du=(h * d).astype(np.float32))
+            jm = functional(case, args.steps, args.last, args.y0, args.y1, du=(-h * d).astype(np.float32))
+            fd.append((jp - jm) / (2 * h))
+        out["fd_h"] = h
+        out["fd"] = fd
+        out["fd_rel"] = [abs(p - f) / abs(f) if f != 0 else None for p, f in zip(out["projections"], fd)]
+        out["fd_max_diff_over_max_fd"] = float(np.max(np.abs(np.array(out["projections"]) - np.array(fd)))
+                                               / np.max(np.abs(fd)))
     print(json.dumps(out))
     return 0
 
