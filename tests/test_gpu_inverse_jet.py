@@ -14,7 +14,15 @@ Tolerance: |adjoint - FD| <= 1e-4 x the largest |FD| over the 8 directions.
 The FP32 error is an absolute floor (about 5e-8 here, 3e-5 of the typical
 directional derivative); one direction's projection cancels to 1/13 of the
 others, so its own relative error is about 3.5e-4.
+
+MCG_LONG=1 adds the full-size problem (128x256x128, R = 8, 2000 steps, y in
+[150, 200], last 500 steps), one test per direction: 2 FP64 oracle runs of
+2000 steps each (about 150 s apiece on 16 host threads).  MCG_LONG_DIRECTIONS
+(e.g. "0,1") selects directions; tolerance 1e-4 of the largest adjoint
+projection.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -30,16 +38,19 @@ def small_case():
     return ij.inverse_jet_case(32, 64, 32, jet_radius=4.0)
 
 
-def oracle_functional(case, du):
-    w = ij.centreline_weights(case, Y0, Y1, LAST).astype(np.float64)
+def oracle_functional(case, du, steps=STEPS, last=LAST, y0=Y0, y1=Y1, tag=None):
+    w = ij.centreline_weights(case, y0, y1, last).astype(np.float64)
+    sel = w != 0
     o = Oracle(case)
     o.init_jet()
     o.set_inlet_offset(du)
     J = 0.0
-    for n in range(STEPS):
+    for n in range(steps):
         o.step(1)  # after it the moment arrays hold the moments pass of step n
-        if n >= STEPS - LAST:
-            J += float(np.sum(w * o.fields()["uy"]))
+        if n >= steps - last:
+            J += float(np.sum(w[sel] * o.fields()["uy"][sel]))
+        if tag and n % 250 == 249:
+            print(f"{tag}: oracle step {n + 1}/{steps}", flush=True)
     o.close()
     return J
 
@@ -72,3 +83,31 @@ def test_inlet_gradient_matches_fp64_fd_on_8_random_directions(grad):
               flush=True)
     adj, fd = np.array(adj), np.array(fd)
     assert np.max(np.abs(adj - fd)) <= 1e-4 * np.max(np.abs(fd)), (adj, fd)
+
+
+FULL = dict(steps=2000, last=500, y0=150, y1=200)
+LONG = os.environ.get("MCG_LONG") == "1"
+LONG_DIRECTIONS = [int(k) for k in os.environ.get("MCG_LONG_DIRECTIONS", "0,1,2,3,4,5,6,7").split(",") if k]
+
+
+@pytest.fixture(scope="module")
+def full_grad():
+    case = ij.inverse_jet_case(128, 256, 128, jet_radius=8.0)
+    print(f"inverse jet {case.NX}x{case.NY}x{case.NZ} steps={FULL['steps']}: adjoint", flush=True)
+    g = ij.gradient(case, FULL["steps"], FULL["last"], FULL["y0"], FULL["y1"], checkpoint_every=200)
+    dirs = ij.random_directions(case, 8, seed=5)
+    return case, dirs, [float(np.sum(g["dinlet"] * d)) for d in dirs]
+
+
+@pytest.mark.skipif(not LONG, reason="full-size FP64 differences (minutes per direction): set MCG_LONG=1")
+@pytest.mark.parametrize("k", range(8))
+def test_full_size_direction_against_fp64_fd(full_grad, k):
+    if k not in LONG_DIRECTIONS:
+        pytest.skip("direction not selected in MCG_LONG_DIRECTIONS")
+    case, dirs, proj = full_grad
+    h = 1e-4
+    fd = (oracle_functional(case, h * dirs[k], tag=f"direction {k} +h", **FULL)
+          - oracle_functional(case, -h * dirs[k], tag=f"direction {k} -h", **FULL)) / (2 * h)
+    print(f"direction {k}: adjoint {proj[k]:.10e} fd {fd:.10e} diff/max {abs(proj[k] - fd) / max(map(abs, proj)):.2e}",
+          flush=True)
+    assert abs(proj[k] - fd) <= 1e-4 * max(map(abs, proj)), (proj[k], fd)
