@@ -273,7 +273,10 @@ struct Store16 {
             return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(bits_f32(f32_bits(lo) + d),
                                                                            bits_f32(f32_bits(hi) + d)));
         } else {
-            const uint32_t d = r >> 16;
+            // d = r >> 16 as a byte permute (selector 0x0c: a zero byte): the
+            // compiler would otherwise fold the shift into word-select (SDWA)
+            // adds, which ran the BF16 step 4.5% slower
+            const uint32_t d = __builtin_amdgcn_perm(0u, r, 0x0c0c0302u);
             // bytes 2,3 of (lo + d), then bytes 2,3 of (hi + d)
             return __builtin_amdgcn_perm(f32_bits(hi) + d, f32_bits(lo) + d, 0x07060302u);
         }
