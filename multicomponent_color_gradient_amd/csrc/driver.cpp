@@ -8,6 +8,7 @@
 // usage: mcg_jet [--reference] [--nx N --ny N --nz N] [--radius R] [--velocity U]
 //                [--Re Re] [--We We] [--steps N] [--noutput N] [--stats-start N]
 //                [--stats-sample N] [--out DIR] [--no-vti] [--kernel fused|twopass]
+//                [--storage fp32|fp16|bf16]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -37,6 +38,7 @@ int main(int argc, char **argv) {
     std::string out = ".";
     bool reference = false;
     int kernel = MCG_KERNEL_FUSED;
+    int storage = MCG_STORE_FP32;
     double R = -1, U = -1, Re = -1, We = -1;
     for (int a = 1; a < argc; ++a) {
         const std::string k = argv[a];
@@ -62,6 +64,16 @@ int main(int argc, char **argv) {
         else if (k == "--out") out = val();
         else if (k == "--no-vti") c.write_vti_output = 0;
         else if (k == "--kernel") kernel = std::string(val()) == "twopass" ? MCG_KERNEL_TWOPASS : MCG_KERNEL_FUSED;
+        else if (k == "--storage") {
+            const std::string v = val();
+            if (v == "fp32") storage = MCG_STORE_FP32;
+            else if (v == "fp16") storage = MCG_STORE_FP16;
+            else if (v == "bf16") storage = MCG_STORE_BF16;
+            else {
+                std::fprintf(stderr, "--storage must be fp32, fp16 or bf16\n");
+                return 2;
+            }
+        }
         else {
             std::fprintf(stderr, "unknown option %s\n", k.c_str());
             return 2;
@@ -93,7 +105,7 @@ int main(int argc, char **argv) {
     TRY("tke files", mcg_tke_files_reset(mean_dir));
 
     mcg_solver *s = nullptr;
-    TRY("create", mcg_create(&c, MCG_STORE_FP32, nullptr, &s));
+    TRY("create", mcg_create(&c, storage, nullptr, &s));
     TRY("kernel", mcg_set_kernel(s, kernel));
     TRY("init", mcg_init_jet(s));
     std::printf("%s | %dx%dx%d | %d steps | out %s\n", mcg_version(), c.NX, c.NY, c.NZ, c.NSTEP, case_dir);

@@ -65,3 +65,26 @@ def test_driver_run_matches_oracle(tmp_path):
     v = mio.read_vti(str(case_dir / "vti_data" / "lbm_00000200.vti"))
     f = o.fields()
     assert np.max(np.abs(v["rhob"] - f["rhob"])) <= 5e-5
+
+
+def test_driver_with_fp16_storage(tmp_path):
+    """mcg_jet --storage fp16: the same short run completes and its radial
+    statistics stay within the 16-bit tolerance of the FP32 run's."""
+    base = ["--nx", "34", "--ny", "46", "--nz", "30", "--radius", "5", "--Re", "400", "--We", "2000",
+            "--steps", "101", "--noutput", "100", "--stats-start", "50", "--no-vti"]
+    got = {}
+    for st in ("fp32", "fp16"):
+        out = tmp_path / st
+        print("driver:", st, flush=True)
+        res = subprocess.run([DRIVER, *base, "--storage", st, "--out", str(out)], capture_output=True, text=True,
+                             timeout=120)
+        assert res.returncode == 0, res.stdout + res.stderr
+        got[st] = post.read_radial_moments(out / "JET_VTK" / "Re400_We2000" / "mean_profiles")
+    assert np.array_equal(got["fp16"]["count"], got["fp32"]["count"])
+    mask = got["fp32"]["count"] > 0
+    a = got["fp16"]["sum_uy"][mask] / got["fp16"]["count"][mask]
+    b = got["fp32"]["sum_uy"][mask] / got["fp32"]["count"][mask]
+    assert np.max(np.abs(a - b)) <= 1e-3 * np.max(np.abs(b))
+    res = subprocess.run([DRIVER, *base, "--storage", "fp8", "--out", str(tmp_path / "x")], capture_output=True,
+                         text=True, timeout=60)
+    assert res.returncode == 2
