@@ -231,8 +231,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this solver runs on one GPU; multi-GPU runs are not part of this build "
-              "(a y-slab decomposition is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: this solver runs on one GPU; a multi-GPU run (y-slab decomposition with halo "
+              "exchange over RCCL) is not implemented, so only --gpus 1 is measured", file=sys.stderr)
         sys.exit(2)
 
     import multicomponent_color_gradient_amd as m
