@@ -221,7 +221,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
-    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-offset comparison run")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16- and BF16-offset comparison runs")
     ap.add_argument("--no-interface-rich", action="store_true", help="skip the interface-rich (256 spheres) run")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=200)
@@ -269,8 +269,9 @@ def main():
     r = measure(args.storage)
     mlups, mlups_int, avg_ms, wall = r["mlups"], r["mlups_int"], r["avg_ms"], r["wall"]
     bpl, achieved, traffic = r["bpl"], r["achieved"], r["traffic"]
-    # the FP16-offset comparison of the metric, on the same workload
+    # the FP16- and BF16-offset comparisons of the metric, on the same workload
     r16 = measure("fp16") if args.storage == "fp32" and not args.no_fp16 else None
+    rb16 = measure("bf16") if args.storage == "fp32" and not args.no_fp16 else None
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib << 20)
 
@@ -300,13 +301,15 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_lu": bpl, "kernel_ms": round(avg_ms, 5)},
     }
-    if r16 is not None:
-        out["fp16_offset"] = {"value": round(r16["mlups"], 2), "unit": "MLUPS", "ms_per_step": round(r16["avg_ms"], 5),
-                              "speedup_vs_fp32": round(r16["mlups"] / mlups, 3), "windows_ms_per_step": r16["windows"],
-                              "mlups_median_window": round(case.NX * case.NY * case.NZ / r16["median_ms"] / 1e3, 2),
-                              "roofline": {"achieved": round(r16["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                           "frac": round(r16["achieved"] / HBM_PEAK_GBS, 4), "traffic": r16["traffic"],
-                                           "bytes_per_lu": r16["bpl"]}}
+    for key, rr in (("fp16_offset", r16), ("bf16_offset", rb16)):
+        if rr is None:
+            continue
+        out[key] = {"value": round(rr["mlups"], 2), "unit": "MLUPS", "ms_per_step": round(rr["avg_ms"], 5),
+                    "speedup_vs_fp32": round(rr["mlups"] / mlups, 3), "windows_ms_per_step": rr["windows"],
+                    "mlups_median_window": round(case.NX * case.NY * case.NZ / rr["median_ms"] / 1e3, 2),
+                    "roofline": {"achieved": round(rr["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(rr["achieved"] / HBM_PEAK_GBS, 4), "traffic": rr["traffic"],
+                                 "bytes_per_lu": rr["bpl"]}}
     if not args.no_interface_rich and args.storage == "fp32" and args.workload == "jet256":
         out["interface_rich"] = interface_rich(args.steps, args.warmup, args.kernel)
     if not args.no_adjoint and args.storage == "fp32":
