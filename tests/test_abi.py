@@ -80,6 +80,8 @@ def test_errors_without_gpu_are_status_codes():
     lib = m.load_library()
     assert lib.mcg_step(None, 1) == _abi.MCG_E_INVALID
     assert lib.mcg_destroy(None) == 0
+    assert lib.mcg_adjoint_set_checkpoint_limit(None, 3) == _abi.MCG_E_INVALID
+    assert lib.mcg_adjoint_recomputed_steps(None, None) == _abi.MCG_E_INVALID
     with pytest.raises(_abi.McgError):
         _abi.check(_abi.MCG_E_INVALID)
 

@@ -188,3 +188,12 @@ def test_torch_jet_steps_with_a_state_cap():
     assert float(A.grad) == dA and float(beta.grad) == dbeta
     assert float(U.grad) == dU and float(amp.grad) == damp
     assert np.array_equal(dut.grad.numpy(), din)
+
+
+def test_checkpoint_limit_misuse():
+    case, fr, fb, _ = make_state()
+    s = m.Solver(case)
+    with pytest.raises(m.McgError):
+        s.adjoint_set_checkpoint_limit(-1)
+    assert s.adjoint_recomputed_steps() == 0  # no sweep yet
+    s.close()
