@@ -293,3 +293,38 @@ def test_uninitialised_step_is_an_error(mcg):
     s = mcg.Solver(case)
     with pytest.raises(mcg.McgError):
         s.step(1)
+
+
+def test_reference_case_matches_oracle(mcg):
+    """The reference's own case (constants.cuh: 128x400x128 jet, D = 14,
+    Re 5000 so tau = 0.50042, We 2000, beta 0.8, sharp inlet), statistics from
+    step 100.  After 20 steps within the parity tolerances of the small cases;
+    after 200 steps (measured: rho 2.2e-5, u 1.8e-6, radial sum_uy 1.0e-4,
+    mass 1.3e-6) within a few times the measured error."""
+    from oracle.oracle import Oracle
+
+    c = mcg.case.reference_jet()
+    c.NSTATS_START_STEP = 100
+    inner = (slice(1, -1),) * 3
+    for steps, d_rho, d_u in ((20, 5e-6, 1e-6), (200, 5e-5, 5e-6)):
+        print(f"reference case {c.NX}x{c.NY}x{c.NZ}, {steps} steps", flush=True)
+        s = mcg.Solver(c)
+        s.init_jet()
+        s.step(steps)
+        g, rg, gs = s.fields(), s.radial(), s.global_sums()
+        s.close()
+        o = Oracle(c)
+        o.init_jet()
+        o.step(steps)
+        r, ro, os_ = o.fields(), o.radial(), o.global_sums()
+        o.close()
+        for k, tol in (("rhor", d_rho), ("rhob", d_rho), ("ux", d_u), ("uy", d_u), ("uz", d_u)):
+            e = float(np.max(np.abs(g[k][inner].astype(np.float64) - r[k][inner])))
+            assert e <= tol, f"{steps} steps {k}: {e:.3e}"
+        assert np.array_equal(rg["count"], ro["count"])
+        mass = float(abs((gs[0] + gs[1]) - (os_[0] + os_[1])) / (os_[0] + os_[1]))
+        assert mass <= 5e-6, mass
+        if steps > c.NSTATS_START_STEP:
+            mask = ro["count"] > 0
+            rel = float(np.max(np.abs(rg["sum_uy"][mask] - ro["sum_uy"][mask])) / np.max(np.abs(ro["sum_uy"][mask])))
+            assert rel <= 5e-4, rel
