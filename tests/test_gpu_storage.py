@@ -111,10 +111,11 @@ def mean_profile_errors(mcg, storage, steps=120):
     return prof, mass
 
 
-@pytest.mark.parametrize("storage,bound", [("fp32", 1e-4), ("fp16", 1e-3), ("bf16", 1e-2)])
+@pytest.mark.parametrize("storage,bound", [("fp32", 1e-4), ("fp16", 1e-3), ("bf16", 2e-3)])
 def test_jet_mean_profile_error(mcg, storage, bound):
     """Mean u_y profiles of the jet: FP16-offset storage within 1e-3 of the FP64
-    oracle (BF16 within 1e-2, FP32 within 1e-4); total mass within 1e-5."""
+    oracle (BF16 within 2e-3, FP32 within 1e-4; measured with the stochastic
+    rounding of the stores: 5e-6, 5e-5, 6e-4); total mass within 1e-5."""
     prof, mass = mean_profile_errors(mcg, storage)
     assert prof <= bound, f"{storage}: mean-profile error {prof:.3e}"
     assert mass <= 1e-5, f"{storage}: mass error {mass:.3e}"
