@@ -98,3 +98,43 @@ def test_centroid_gradient_over_1000_steps():
     assert abs(ref["dA"] - fd_a) <= 1e-4 * abs(fd_a), (ref["dA"], fd_a)
     for rel in (1e-3, 1e-5):
         print(f"dgravity: FP64 FD (h = {rel:g}) {fd('gravity', rel):.10e}", flush=True)
+
+
+@pytest.mark.skipif(__import__("os").environ.get("MCG_LONG") != "1",
+                    reason="96x192x96 over 2000 steps against FP64 differences (minutes of host time): set MCG_LONG=1")
+def test_centroid_dA_96_over_2000_steps():
+    """96x192x96 (R = 8 at y = 32; gravity 1e-5, tau 0.55/1.0, A = 1e-3 as the
+    full-size case), 2000 steps, checkpoints every 200: the GPU adjoint's dJ/dA
+    against central differences of the FP64 oracle at h = 1e-5 relative,
+    within 1e-4.  (dJ/dgravity differences are not a check here: J(gravity) is
+    only piecewise smooth, see test_centroid_gradient_over_1000_steps.)"""
+    from oracle.oracle import Oracle
+
+    c = m.rising_bubble_case(96, 192, 96, radius=8.0, y0=32.0)
+    steps = 2000
+    print(f"rising centroid {c.NX}x{c.NY}x{c.NZ} steps={steps}: adjoint", flush=True)
+    s = m.Solver(c)
+    s.init_bubble_tanh()
+    s.adjoint_record(steps, 200)
+    fb = s.populations()[1]
+    lam_b = centroid_seed(fb)
+    s.adjoint_backward(np.zeros_like(lam_b), lam_b)
+    g = s.adjoint_gradients()
+    s.close()
+
+    def oracle_J(value):
+        cc = c.copy()
+        cc.A_override = value
+        o = Oracle(cc)
+        o.init_bubble_tanh()
+        for k in range(steps // 500):
+            o.step(500)
+            print(f"oracle A={value:.8g}: step {500 * (k + 1)}/{steps}", flush=True)
+        J = centroid_height(o.populations()[1])
+        o.close()
+        return J
+
+    a0 = c.A_override
+    fd = (oracle_J(a0 * (1 + 1e-5)) - oracle_J(a0 * (1 - 1e-5))) / (2e-5 * a0)
+    print(f"dA: adjoint {g['dA']:.10e} FP64 FD (h = 1e-5) {fd:.10e} rel {abs(g['dA'] - fd) / abs(fd):.2e}", flush=True)
+    assert abs(g["dA"] - fd) <= 1e-4 * abs(fd), (g["dA"], fd)
