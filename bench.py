@@ -207,7 +207,31 @@ def cpu_baseline(case, init: str, budget_s: float):
         "kind": "port",
         "sample": f"FP64 OpenMP oracle, {case.NX}x{case.NY}x{case.NZ}, 1 warm-up + {n} timed steps "
                   f"({dt:.1f} s, setup {t_setup:.1f} s)",
+        "other_grids": cpu_other_grids(),
     }
+
+
+def cpu_other_grids(budget_s: float = 2.0) -> dict:
+    """The oracle's MLUPS on the two smaller configurations: the 64^3 static
+    bubble and the 128x256x128 jet (1 warm-up step, then steps for about
+    budget_s each)."""
+    import multicomponent_color_gradient_amd as m
+    from oracle import oracle as orc
+
+    out = {}
+    for name, c, init in (("static bubble 64^3", m.periodic_bubble_case(64, radius=12.0, tau=0.8), "bubble_tanh"),
+                          ("jet 128x256x128", m.perturbed_jet_case(128, 256, 128, jet_radius=8.0), "jet")):
+        o = orc.Oracle(c)
+        getattr(o, "init_" + init)()
+        o.step(1)
+        n, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < budget_s and n < 200:
+            o.step(1)
+            n += 1
+        dt = time.perf_counter() - t1
+        o.close()
+        out[name] = {"mlups": round(n * c.NX * c.NY * c.NZ / dt / 1e6, 3), "steps": n}
+    return out
 
 
 def main():
