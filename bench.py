@@ -114,7 +114,8 @@ def adjoint_ratio(steps: int, every: int) -> dict:
     return {"workload": "rising bubble 192x384x192, walls x/z, R=16 at y=64, gravity 1e-5, tau 0.55/1.0, A=1e-3, "
                         "beta 0.7; centroid-height objective", "steps": steps,
             "checkpoint_every": every, "forward_ms": round(fwd_ms, 3), "backward_ms": round(bwd_ms, 3),
-            "ratio": round(bwd_ms / fwd_ms, 3)}
+            "ratio": round(bwd_ms / fwd_ms, 3),
+            "forward_mlups": round(steps * c.NX * c.NY * c.NZ / (fwd_ms / 1e3) / 1e6, 2)}
 
 
 def adjoint_jet_ratio(steps: int, every: int) -> dict:
@@ -141,7 +142,8 @@ def adjoint_jet_ratio(steps: int, every: int) -> dict:
     s.close()
     return {"workload": "perturbed jet 128x256x128, R=8, U 0.05, tanh inlet, perturbation 0.01", "steps": steps,
             "checkpoint_every": every, "forward_ms": round(fwd_ms, 3), "backward_ms": round(bwd_ms, 3),
-            "ratio": round(bwd_ms / fwd_ms, 3)}
+            "ratio": round(bwd_ms / fwd_ms, 3),
+            "forward_mlups": round(steps * c.NX * c.NY * c.NZ / (fwd_ms / 1e3) / 1e6, 2)}
 
 
 def interface_rich(steps: int, warmup: int, kernel: str) -> dict:
