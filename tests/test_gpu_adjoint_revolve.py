@@ -197,3 +197,26 @@ def test_checkpoint_limit_misuse():
         s.adjoint_set_checkpoint_limit(-1)
     assert s.adjoint_recomputed_steps() == 0  # no sweep yet
     s.close()
+
+
+def test_short_last_segment():
+    """25 steps with K = 4: seven segments, the last of one step; limits 3 and
+    1 give the unlimited record's gradients bit for bit and the schedule's
+    recomputation counts."""
+    case, fr, fb, _ = make_state()
+    steps, every = 25, 4
+    rng = np.random.default_rng(8)
+    lam = [rng.standard_normal((27,) + fr.shape[1:]).astype(np.float32) for _ in range(2)]
+    res = {}
+    for limit in (0, 3, 1):
+        print(f"rising bubble {case.NX}x{case.NY}x{case.NZ}: {steps} steps, every {every}, limit {limit}", flush=True)
+        s = m.Solver(case)
+        s.init_bubble_tanh()
+        s.set_populations(fr.astype(np.float32), fb.astype(np.float32))
+        s.adjoint_set_checkpoint_limit(limit)
+        s.adjoint_record(steps, every)
+        res[limit] = sweep(s, lam)
+        assert res[limit][3] == schedule_cost(steps, every, limit), (limit, res[limit][3])
+        s.close()
+    for limit in (3, 1):
+        same(res[limit], res[0])
