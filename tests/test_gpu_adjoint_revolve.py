@@ -6,6 +6,7 @@ so every gradient must be bitwise that of the unlimited record:
     segments of 2, limits 6, 3, 2, 1;
   * jet (inlet/outlet arrays, outlet state, u_y seed, inlet gradient), 7 steps
     in 7 segments, limits 3, 1;
+  * rising bubble, 25 steps in segments of 4 (the last of one step), limits 3, 1;
 and a second sweep over the same record gives the same result.  The number of
 recomputed forward steps equals that of the schedule restated below, and the
 unlimited record recomputes each step once.
