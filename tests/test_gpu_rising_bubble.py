@@ -67,3 +67,19 @@ def test_rising_bubble_fp16_close_to_fp32():
         res.append(s.fields())
     assert np.max(np.abs(res[1]["uy"] - res[0]["uy"])) < 2e-5
     assert np.max(np.abs(res[1]["rhob"] - res[0]["rhob"])) < 2e-3
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_rising_bubble_16bit_fused_equals_twopass(storage):
+    """Walls and the body force (the EXT instance of the fused kernel) with
+    16-bit storage: the fused and two-pass kernels store the same stochastically
+    rounded words (same random stream per node and step), bit for bit."""
+    c = small_case()
+    out = []
+    for kernel in ("fused", "twopass"):
+        s = m.Solver(c, storage=storage, kernel=kernel)
+        s.init_bubble_tanh()
+        s.step(30)
+        out.append(s.populations())
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
