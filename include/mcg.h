@@ -48,7 +48,7 @@ extern "C" {
 #define MCG_VISC_INTERFACE 1 /* tau = (1+psi)/2 taur + (1-psi)/2 taub per node (forces.cuh:77-85)   */
 
 /* inlet velocity profile (MCG_Y_JET only) */
-#define MCG_INLET_SHARP 0 /* top-hat disk, jet/boundary_conditions.cuh:97-111                        */
+#define MCG_INLET_SHARP 0 /* top-hat disk, jet/boundary_conditions.cuh:33-47                         */
 #define MCG_INLET_TANH 1  /* tanh-edged top-hat plus seeded transverse perturbation (extension)       */
 
 /* population storage */
@@ -294,7 +294,7 @@ int mcg_uy_functional(mcg_solver *s, double *value);
  * entries of non-jet columns are ignored; NULL removes it).  The control of
  * the inverse jet problem: mcg_adjoint_inlet_gradient returns dJ/d(du).  A
  * solver setting, not part of a checkpoint.  No counterpart in the reference
- * (an extension of jet/boundary_conditions.cuh:62-75). */
+ * (an extension of jet/boundary_conditions.cuh:45-47). */
 int mcg_set_inlet_offset(mcg_solver *s, const float *du);
 
 /* Jet cases, after mcg_adjoint_backward: dparams = {dJ/djet_velocity,

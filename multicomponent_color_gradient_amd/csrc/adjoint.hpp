@@ -336,7 +336,7 @@ __device__ __forceinline__ void block_part2(double a, double b, double *part) {
 //    this step's outlet state gBn, from lgB_in = adjoint of the next one;
 //  * psi of the outlet ghost plane, read by the colour gradient of row NY-2:
 //    its transpose, into lgB_out;
-//  * inlet of a jet column (jet/boundary_conditions.cuh:97-148): adds to
+//  * inlet of a jet column (jet/boundary_conditions.cuh:17-85): adds to
 //    dJ/de of the column (FP64, summed over steps) and the dU, damp partials
 //    part[2 b], part[2 b + 1].
 __global__ __launch_bounds__(256) void k_adj_cols(DevParams P, const float *__restrict__ mom,

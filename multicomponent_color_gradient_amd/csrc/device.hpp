@@ -15,11 +15,11 @@
 //                  population that the ghost node (x,0,z) / (x,NY-1,z) sends
 //                  into the first/last fluid plane, double buffered.  Entries
 //                  the reference never writes keep their initial values
-//                  (quirks Q1/Q2: jet/boundary_conditions.cuh:102-105 and the
-//                  unwrapped targets of :137,:286).
+//                  (quirks Q1/Q2: jet/boundary_conditions.cuh:38-41 and the
+//                  unwrapped targets of :73,:222).
 //   outlet ghost : [5][NZ][NX] rho_r, rho_b, u_x, u_y, u_z of plane y=NY-1,
 //                  double buffered (the colour gradient of step n reads the
-//                  values of step n-1, jet/boundary_conditions.cuh:256-260).
+//                  values of step n-1, jet/boundary_conditions.cuh:186-190).
 //   ghost rho    : full-grid rho_r, rho_b holding the values of ghost nodes
 //                  (x/z frames and the y=0 plane), read by the colour
 //                  gradient without wrap (forces.cuh:31-33).
@@ -317,7 +317,7 @@ __device__ __forceinline__ Src source_of(const DevParams &P, int x, int y, int z
         if (P.y_mode == MCG_Y_JET) {
             const bool ghost = kCY[I] > 0 ? (ys == 0) : (ys == P.NY - 1);
             if (ghost) {
-                // ghost-plane sources are not wrapped (jet/boundary_conditions.cuh:137,286)
+                // ghost-plane sources are not wrapped (jet/boundary_conditions.cuh:73,222)
                 const size_t nxz = size_t(P.NX) * size_t(P.NZ);
                 const size_t off = size_t(slot_of(I)) * 2 * nxz + size_t(zs) * P.NX + size_t(xs);
                 return Src{kCY[I] > 0 ? 1 : 2, off};
@@ -548,7 +548,7 @@ __device__ __forceinline__ float inlet_xi(uint32_t seed, uint64_t step, int x, i
 }
 
 // Inlet velocity of jet ghost node (x,0,z).  Sharp profile: (0, U, 0)
-// (jet/boundary_conditions.cuh:62-75).  MCG_INLET_TANH (an extension beyond
+// (jet/boundary_conditions.cuh:45-47).  MCG_INLET_TANH (an extension beyond
 // the reference): u_y = U (1 - tanh((r - R)/w))/2 on the jet disk, and
 // transverse u_x, u_z = amp U (2 xi - 1) with xi drawn per step and column.
 __device__ __forceinline__ void inlet_velocity(const DevParams &P, const float *du, uint64_t step, int x, int z,
@@ -574,7 +574,7 @@ __device__ __forceinline__ void inlet_velocity(const DevParams &P, const float *
 }
 
 // Inlet populations sent by ghost node (x,0,z) of a jet column, computed from
-// the moments m of node (x,1,z): jet/boundary_conditions.cuh:97-148.
+// the moments m of node (x,1,z): jet/boundary_conditions.cuh:17-85.
 __device__ __forceinline__ void inlet_emit(const DevParams &P, float *__restrict__ in_dst, const Moments &m, int x, int z,
                                            uint64_t step, const float *du) {
     const float rrB = (1.0f - 1.0f) * P.rhor0;  // jet node: is_jet = 1

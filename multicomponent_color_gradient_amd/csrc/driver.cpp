@@ -127,7 +127,7 @@ int main(int argc, char **argv) {
         step = last + 1;  // steps 0..last done
         int32_t started = 0;
         int64_t start_step = 0, samples = 0;
-        mcg_stats_info(s, &started, &start_step, &samples);
+        TRY("statistics state", mcg_stats_info(s, &started, &start_step, &samples));
         if (started && !stats_announced) {
             std::printf("Starting radial temporal statistics at step %lld | stats_start_step = %d\n", (long long)start_step,
                         c.NSTATS_START_STEP);

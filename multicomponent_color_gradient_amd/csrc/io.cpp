@@ -6,8 +6,6 @@
 //   .../vti_data/lbm_%08d.vti (VTK XML ImageData, base64)   io/save_data.cuh:22-215
 // The byte layouts are the reference's so that its post-processing reads
 // them unchanged; radial_sum_phi{,2}.bin are additional files.
-#include <hip/hip_runtime.h>
-
 #include <cerrno>
 #include <cmath>
 #include <cstdint>

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_init_ghost(DevParams P, int kind, InitG
 }
 
 // the inlet rewrites the ghost densities of plane y=0 every step
-// (jet/boundary_conditions.cuh:97-100); they only change on the first step
+// (jet/boundary_conditions.cuh:35-36); they only change on the first step
 __global__ __launch_bounds__(256) void k_inlet_ghost_reset(DevParams P, float *grr, float *grb) {
     const int x = 1 + int(blockIdx.x) * 256 + int(threadIdx.x);
     const int z = 1 + int(blockIdx.y);

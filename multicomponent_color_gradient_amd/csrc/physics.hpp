@@ -256,8 +256,8 @@ __device__ __forceinline__ void collide_pair_w(const NodeColl &n, float &ap, flo
     bm = fma_(n.aB, gm, D);
 }
 
-// Regularised boundary population (inlet jet/boundary_conditions.cuh:128-147,
-// outlet :278-296): g = feq(rho, u_eq) + (1-omega) fneqr(Pi, u_neq).
+// Regularised boundary population (inlet jet/boundary_conditions.cuh:76-79,
+// outlet :225-227): g = feq(rho, u_eq) + (1-omega) fneqr(Pi, u_neq).
 template <int I>
 __device__ __forceinline__ float boundary_pop(float rho, float ex, float ey, float ez, float oms, const Moments &mn) {
     return feq<I>(rho, ex, ey, ez) + oms * fneqr<I>(mn);

@@ -2,7 +2,7 @@
 respect to the inlet velocity of every jet column, by the GPU adjoint.
 
 The control is a per-column offset du (3 x NZ x NX, component-major u_x, u_y,
-u_z) added to the inlet velocity of jet/boundary_conditions.cuh:62-75 (here
+u_z) added to the inlet velocity of jet/boundary_conditions.cuh:45-47 (here
 with the tanh profile and seeded transverse perturbation of the perturbed-jet
 case).  The objective is
 

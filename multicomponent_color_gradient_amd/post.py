@@ -5,13 +5,18 @@ Restates the reference's offline consumers of ``mean_profiles/``:
   fit_virtual_origin         (post/mean_profile_io.py:61-239)
   first_half_radius, radial_bin_centers, centerline_summary
                              (post/plot_profiles.py:53-92,120-135,276-308)
-Plotting is not reproduced.  Numerics use numpy; results depend on numpy's
-least-squares fit (np.polyfit), so they are pinned by hand-computed cases,
-not against the reference's own outputs (whose inputs are not shipped).
+Plotting is not reproduced.  The line fit is solved in closed form here (no
+library fit routine), and every function is pinned by hand-computed cases and
+the one reference-held number (the Hussein half-radius slope of
+resultsJET/profile_summary.txt:7), not against the reference's own outputs,
+whose inputs are not shipped.
 """
 from __future__ import annotations
 
+import ast
+import re
 from pathlib import Path
+from typing import NamedTuple
 
 import numpy as np
 
@@ -38,14 +43,125 @@ HUSSEIN_U = np.array([
 
 def read_metadata(mean_dir) -> dict:
     """key value pairs of radial_profile_metadata.txt (post/mean_profile_io.py:61-100)."""
-    p = Path(mean_dir) / "radial_profile_metadata.txt"
-    md: dict = {}
+    return _read_key_values(Path(mean_dir) / "radial_profile_metadata.txt")
+
+
+def _read_key_values(path) -> dict:
+    """'key value' lines of a metadata file, with the known integer and float
+    keys converted; {} when the file does not exist."""
+    p = Path(path)
     if not p.exists():
-        return md
+        return {}
+    md: dict = {}
     for line in p.read_text().splitlines():
-        parts = line.split(maxsplit=1)
-        if len(parts) == 2:
-            md[parts[0]] = parts[1]
+        key, _, value = line.strip().partition(" ")
+        if key and value.strip():
+            md[key] = value.strip()
+    for keys, conv in ((INT_KEYS, int), (FLOAT_KEYS, float)):
+        for k in keys:
+            if k in md:
+                md[k] = conv(md[k])
+    return md
+
+
+# ---- case constants when a run left no metadata --------------------------
+# The reference's post scripts fall back to its compile-time constants.cuh
+# (post/mean_profile_io.py:21-58, post/case_config.py:38-76).  Here the
+# fallback chain is: radial_profile_metadata.txt, then case_info.txt of the
+# case folder (written by the driver), then a constants.cuh-style file whose
+# declarations are evaluated in order (so derived values such as
+# jet_x0 = (NX - 1)/2 resolve too).
+
+_DECL = re.compile(r"^\s*(?:inline\s+)?(?:static\s+)?(?:constexpr|const)\s+[\w:<>]+\s+(\w+)\s*=\s*(.+?);", re.M)
+_CAST = re.compile(r"static_cast\s*<\s*([\w:]+)\s*>")
+_FLOAT_TYPES = ("float", "double", "real_t", "pop_t")
+
+
+def _eval_expr(expr: str, env: dict):
+    """Value of an arithmetic expression over numbers and earlier names
+    (+ - * /, parentheses and static_cast to a floating or integer type, with
+    C++'s integer division); None when it uses anything else."""
+    src = _CAST.sub(lambda mt: "_float_" if mt.group(1) in _FLOAT_TYPES else "_int_", expr)
+    try:
+        tree = ast.parse(src.replace("std::", ""), mode="eval")
+    except SyntaxError:
+        return None
+
+    def ev(n):
+        if isinstance(n, ast.Expression):
+            return ev(n.body)
+        if isinstance(n, ast.Constant) and isinstance(n.value, (int, float)) and not isinstance(n.value, bool):
+            return n.value
+        if isinstance(n, ast.Name):
+            if n.id in ("true", "false"):
+                return int(n.id == "true")
+            return env[n.id]
+        if isinstance(n, ast.Call) and isinstance(n.func, ast.Name) and n.func.id in ("_float_", "_int_") \
+                and len(n.args) == 1 and not n.keywords:
+            v = ev(n.args[0])
+            return float(v) if n.func.id == "_float_" else int(v)
+        if isinstance(n, ast.UnaryOp) and isinstance(n.op, (ast.USub, ast.UAdd)):
+            v = ev(n.operand)
+            return -v if isinstance(n.op, ast.USub) else v
+        if isinstance(n, ast.BinOp) and isinstance(n.op, (ast.Add, ast.Sub, ast.Mult, ast.Div)):
+            a, b = ev(n.left), ev(n.right)
+            if isinstance(n.op, ast.Add):
+                return a + b
+            if isinstance(n.op, ast.Sub):
+                return a - b
+            if isinstance(n.op, ast.Mult):
+                return a * b
+            if isinstance(a, int) and isinstance(b, int):
+                return int(a / b)  # C++ integer division truncates toward zero
+            return a / b
+        raise KeyError(type(n).__name__)
+
+    try:
+        return ev(tree)
+    except (KeyError, ZeroDivisionError, TypeError, ValueError):
+        return None
+
+
+def read_case_constants(path) -> dict:
+    """Evaluate the scalar declarations of a constants.cuh-style header
+    (constants.cuh:8-76): NX, NY, NZ, NOUTPUT, NSTATS_START_STEP, jet_radius,
+    jet_x0, ... in declaration order.  Adds D = 2 jet_radius and U_jet =
+    jet_velocity, the metadata names of the same quantities, and NR_BINS =
+    NX/2 when the file does not declare it.  {} when the file is missing."""
+    p = Path(path)
+    if not p.exists():
+        return {}
+    text = re.sub(r"//[^\n]*", "", p.read_text())
+    env: dict = {}
+    for name, expr in _DECL.findall(text):
+        v = _eval_expr(expr, env)
+        if v is not None:
+            env[name] = v
+    if "jet_radius" in env:
+        env.setdefault("D", 2.0 * float(env["jet_radius"]))
+    if "jet_velocity" in env:
+        env.setdefault("U_jet", float(env["jet_velocity"]))
+    if "NX" in env:
+        env.setdefault("NR_BINS", int(env["NX"]) // 2)
+    return env
+
+
+def case_metadata(mean_dir, constants_file=None) -> dict:
+    """The run's metadata with the fallbacks filled in: keys of
+    radial_profile_metadata.txt win, then case_info.txt in the case folder
+    above mean_dir, then the constants file (if given)."""
+    mean_dir = Path(mean_dir)
+    md = read_metadata(mean_dir)
+    layers = [_read_key_values(mean_dir.parent / "case_info.txt")]
+    if constants_file is not None:
+        layers.append(read_case_constants(constants_file))
+    for layer in layers:
+        if "jet_diameter" in layer:
+            layer.setdefault("D", float(layer["jet_diameter"]))
+        if "jet_velocity" in layer:
+            layer.setdefault("U_jet", float(layer["jet_velocity"]))
+        for k, v in layer.items():
+            md.setdefault(k, v)
     for k in INT_KEYS:
         if k in md:
             md[k] = int(md[k])
@@ -55,17 +171,22 @@ def read_metadata(mean_dir) -> dict:
     return md
 
 
-def read_radial_moments(mean_dir) -> dict:
+def read_radial_moments(mean_dir, constants_file=None) -> dict:
     """The six radial files reshaped to (NY, NR_BINS) (post/mean_profile_io.py:139-179),
-    plus the psi moments when present."""
+    plus the psi moments when present.  NY and NR_BINS come from the run's
+    metadata, else from case_info.txt or the constants file (case_metadata)."""
     mean_dir = Path(mean_dir)
-    md = read_metadata(mean_dir)
-    ny, nr = int(md.get("NY", 0)), int(md.get("NR_BINS", 0))
+    md = case_metadata(mean_dir, constants_file)
+    ny = int(md.get("NY", 0))
+    nr = int(md.get("NR_BINS", int(md.get("NX", 0)) // 2))
     if ny <= 0 or nr <= 0:
-        raise ValueError("NY and NR_BINS missing from radial_profile_metadata.txt")
+        raise ValueError("NY and NR_BINS missing from the run metadata, case_info.txt and the constants file")
 
     def rd(name, dtype):
-        a = np.fromfile(mean_dir / name, dtype=dtype)
+        path = mean_dir / name
+        if not path.exists():
+            raise FileNotFoundError(f"{path} missing: the run wrote no radial statistics yet")
+        a = np.fromfile(path, dtype=dtype)
         if a.size != ny * nr:
             raise ValueError(f"{name} has {a.size} values, expected {ny * nr}")
         return a.reshape(ny, nr)
@@ -80,58 +201,95 @@ def read_radial_moments(mean_dir) -> dict:
     return out
 
 
+def _per_sample(total, count):
+    """total / count per bin; NaN in bins without samples."""
+    c = np.asarray(count, np.float64)
+    has = c > 0
+    q = np.asarray(total, np.float64) / np.where(has, c, 1.0)
+    return np.where(has, q, np.nan)
+
+
+def _spread(mean_sq, mean):
+    """sqrt(<v^2> - <v>^2), with the rounding-negative variance taken as 0."""
+    return np.sqrt(np.clip(mean_sq - np.square(mean), 0.0, None))
+
+
 def averaged_radial_fields(m: dict) -> dict:
-    """Means, rms and Reynolds stress per (y, r-bin) (post/mean_profile_io.py:182-210)."""
-    count = m["count"].astype(np.float64)
-    valid = count > 0
-
-    def avg(name):
-        o = np.full(count.shape, np.nan)
-        np.divide(m[name], count, out=o, where=valid)
-        return o
-
-    uy, uy2, ur, ur2, uruy = (avg(k) for k in ("sum_uy", "sum_uy2", "sum_ur", "sum_ur2", "sum_uruy"))
-    out = {"uy": uy, "uy2": uy2, "ur": ur, "ur2": ur2, "uruy": uruy,
-           "uy_rms": np.sqrt(np.maximum(uy2 - uy * uy, 0.0)), "ur_rms": np.sqrt(np.maximum(ur2 - ur * ur, 0.0)),
-           "uruy_reynolds": uruy - ur * uy, "count": m["count"]}
-    if "sum_phi" in m:
-        out["phi"] = avg("sum_phi")
-        out["phi_rms"] = np.sqrt(np.maximum(avg("sum_phi2") - out["phi"] ** 2, 0.0))
+    """Per (y, r-bin) sample means of the accumulated moments and what they
+    give: rms of u_y and u_r, the Reynolds stress <u_r u_y> - <u_r><u_y>, and
+    with the psi moments the mean and rms of psi.  Bins without samples hold
+    NaN.  The quantities of post/mean_profile_io.py:182-210."""
+    count = m["count"]
+    means = {k[4:]: _per_sample(v, count) for k, v in m.items() if k.startswith("sum_")}
+    out = {k: means[k] for k in ("uy", "uy2", "ur", "ur2", "uruy")}
+    out["uy_rms"] = _spread(means["uy2"], means["uy"])
+    out["ur_rms"] = _spread(means["ur2"], means["ur"])
+    out["uruy_reynolds"] = means["uruy"] - means["ur"] * means["uy"]
+    out["count"] = count
+    if "phi" in means:
+        out["phi"] = means["phi"]
+        out["phi_rms"] = _spread(means["phi2"], means["phi"])
     return out
 
 
-def fit_virtual_origin(y, uc, diameter, u_jet, yi_over_d=10.0, yf_over_d=15.0):
-    """Least-squares line U_jet/U_c = (y/D)/B - y0/(B D) over y/D in [yi, yf]:
-    returns (B, y0, x_fit, y_fit, slope, intercept) (post/mean_profile_io.py:221-239)."""
-    yd = np.asarray(y, np.float64) / diameter
-    uc = np.asarray(uc, np.float64)
-    mask = np.isfinite(uc) & (uc > 0.0) & (yd >= yi_over_d) & (yd <= yf_over_d)
-    if np.count_nonzero(mask) < 2:
+class OriginFit(NamedTuple):
+    """Straight line U_jet/U_c = slope (y/D) + intercept through the
+    centre-line decay, with B = 1/slope and the virtual origin y0 where the
+    line crosses zero; x and inv_uc are the points fitted."""
+    B: float
+    y0: float
+    x: np.ndarray
+    inv_uc: np.ndarray
+    slope: float
+    intercept: float
+
+
+def fit_virtual_origin(y, uc, diameter, u_jet, yi_over_d=10.0, yf_over_d=15.0) -> OriginFit:
+    """Least-squares line through U_jet/U_c against y/D over the samples with
+    finite, positive U_c and yi <= y/D <= yf (the fit of
+    post/mean_profile_io.py:221-239).  The normal equations are solved about
+    the centroid of the points: slope = S_xy/S_xx, intercept = mean(v) -
+    slope mean(x); then B = 1/slope and y0 = -intercept B D."""
+    x = np.asarray(y, np.float64) / diameter
+    u = np.asarray(uc, np.float64)
+    with np.errstate(invalid="ignore"):
+        keep = np.isfinite(u) & (u > 0.0) & (x >= yi_over_d) & (x <= yf_over_d)
+    if np.count_nonzero(keep) < 2:
         raise ValueError("not enough valid centerline samples in the fit range")
-    x_fit = yd[mask]
-    y_fit = u_jet / uc[mask]
-    slope, intercept = np.polyfit(x_fit, y_fit, 1)
+    xs = x[keep]
+    vs = u_jet / u[keep]
+    dx = xs - xs.mean()
+    sxx = float(np.dot(dx, dx))
+    if sxx == 0.0:
+        raise ValueError("the fit range holds a single y position")
+    slope = float(np.dot(dx, vs - vs.mean())) / sxx
+    intercept = float(vs.mean()) - slope * float(xs.mean())
     B = 1.0 / slope
-    y0 = -intercept * B * diameter
-    return B, y0, x_fit, y_fit, slope, intercept
+    return OriginFit(B, -intercept * B * diameter, xs, vs, slope, intercept)
 
 
 def first_half_radius(profile, radius, center_value):
-    """First radius where profile/center drops to 1/2, linearly interpolated
-    (post/plot_profiles.py:53-70)."""
-    if not np.isfinite(center_value) or center_value <= 0.0:
+    """Radius where profile/center_value first falls to 1/2 (the half-radius
+    of post/plot_profiles.py:53-70): the first sample i >= 1 at or below 1/2,
+    linearly interpolated against the sample before it, which must lie above
+    1/2 (a genuine crossing).  NaN when the centre value is not positive, no
+    sample falls to 1/2, or the sample before the first one at or below 1/2
+    is not a finite value above it."""
+    if not (np.isfinite(center_value) and center_value > 0.0):
         return np.nan
     ratio = np.asarray(profile, np.float64) / center_value
-    below = np.where(np.isfinite(ratio) & (ratio <= 0.5))[0]
-    below = below[below > 0]
-    if below.size == 0:
+    with np.errstate(invalid="ignore"):
+        at_or_below = np.isfinite(ratio[1:]) & (ratio[1:] <= 0.5)
+    hit = np.flatnonzero(at_or_below)
+    if hit.size == 0:
         return np.nan
-    i = int(below[0])
-    r1, r2 = radius[i - 1], radius[i]
-    u1, u2 = ratio[i - 1], ratio[i]
-    if not np.isfinite(u1) or not np.isfinite(u2) or np.isclose(u1, u2):
+    k = int(hit[0]) + 1
+    above, below = ratio[k - 1], ratio[k]
+    if not (np.isfinite(above) and above > 0.5):
         return np.nan
-    return r1 + (0.5 - u1) * (r2 - r1) / (u2 - u1)
+    r = np.asarray(radius, np.float64)
+    # np.interp wants increasing abscissae: (below, above) -> (r_k, r_{k-1})
+    return float(np.interp(0.5, [below, above], [r[k], r[k - 1]]))
 
 
 def radial_bin_centers(nx, nz, nr, x0, z0):
@@ -151,10 +309,12 @@ def radial_bin_centers(nx, nz, nr, x0, z0):
 
 
 def centerline_summary(mean_dir, diameter=None, u_jet=None, slices=(10, 11, 12, 13, 14, 15), fit_start=10.0,
-                       fit_end=15.0) -> dict:
+                       fit_end=15.0, constants_file=None) -> dict:
     """Virtual-origin fit and half-radius spreading rates of a run
-    (the numbers of profile_summary.txt, post/plot_profiles.py:112-135,276-308)."""
-    m = read_radial_moments(mean_dir)
+    (the numbers of profile_summary.txt, post/plot_profiles.py:112-135,276-308).
+    D and U_jet default to the run's metadata, then case_info.txt, then the
+    constants file (case_metadata)."""
+    m = read_radial_moments(mean_dir, constants_file)
     md = m["metadata"]
     diameter = float(diameter if diameter is not None else md.get("D", 16.0))
     u_jet = float(u_jet if u_jet is not None else md.get("U_jet", 0.05))

@@ -465,7 +465,7 @@ struct Oracle final : OracleBase {
         return double(splitmix64(key + uint64_t(k)) >> 40) / 16777216.0;
     }
     // Inlet velocity: (0, U, 0) for the reference's sharp profile
-    // (jet/boundary_conditions.cuh:62-75); for MCG_INLET_TANH (an extension)
+    // (jet/boundary_conditions.cuh:45-47); for MCG_INLET_TANH (an extension)
     // u_y = U (1 - tanh((r - R)/w))/2 and u_x, u_z = amp U (2 xi - 1)
     void inlet_velocity(int x, int z, T &ex, T &ey, T &ez) const {
         ex = T(0);

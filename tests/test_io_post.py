@@ -146,3 +146,73 @@ def test_averaged_fields_and_bin_centers():
     assert r[0] == 0.0
     assert r[1] == pytest.approx(np.sqrt(2.5), rel=1e-15)
     assert r[2] == pytest.approx(np.sqrt(4.5), rel=1e-15)
+
+
+def test_fit_virtual_origin_matches_library_least_squares():
+    # the closed-form fit against numpy's least-squares polynomial fit on noisy points
+    rng = np.random.default_rng(11)
+    D, U = 16.0, 0.05
+    y = np.arange(300, dtype=float)
+    uc = U * 6.0 * D / np.maximum(y - 40.0, 1.0) * (1.0 + 0.02 * rng.standard_normal(y.size))
+    uc[170] = np.nan  # skipped: not finite
+    uc[200] = -1.0    # skipped: not positive
+    fit = post.fit_virtual_origin(y, uc, D, U)
+    keep = np.isfinite(uc) & (uc > 0) & (y / D >= 10.0) & (y / D <= 15.0)
+    slope, icpt = np.polyfit(y[keep] / D, U / uc[keep], 1)
+    assert fit.slope == pytest.approx(slope, rel=1e-12) and fit.intercept == pytest.approx(icpt, rel=1e-10)
+    assert fit.B == pytest.approx(1.0 / slope, rel=1e-12)
+    assert fit.y0 == pytest.approx(-icpt / slope * D, rel=1e-10)
+    assert fit.x.size == np.count_nonzero(keep)
+    with pytest.raises(ValueError):
+        post.fit_virtual_origin(y, np.full(y.size, np.nan), D, U)
+
+
+def test_first_half_radius_crossings():
+    r = np.array([0.0, 1.0, 2.0, 3.0])
+    # a crossing exactly on a sample
+    assert post.first_half_radius(np.array([2.0, 1.5, 1.0, 0.2]), r, 2.0) == 2.0
+    # the first crossing wins over a later one
+    assert post.first_half_radius(np.array([1.0, 0.4, 0.9, 0.1]), r, 1.0) == pytest.approx(5.0 / 6.0, rel=1e-14)
+    # a non-finite sample before the first one at or below 1/2: no crossing
+    assert np.isnan(post.first_half_radius(np.array([1.0, np.nan, 0.3, 0.1]), r, 1.0))
+    # the centre sample itself is ignored as a crossing
+    assert np.isnan(post.first_half_radius(np.array([0.4, 0.45, 0.3, 0.1]), r, 1.0))
+
+
+def test_metadata_fallbacks(tmp_path):
+    # no radial_profile_metadata.txt: NY and NR_BINS from case_info.txt, then from a constants header
+    case_dir = tmp_path / "JET_VTK" / "Re100_We10"
+    mean_dir = case_dir / "mean_profiles"
+    mean_dir.mkdir(parents=True)
+    ny, nr = 5, 3
+    rng = np.random.default_rng(4)
+    for k in ("sum_uy", "sum_uy2", "sum_ur", "sum_ur2", "sum_uruy"):
+        rng.standard_normal(ny * nr).tofile(mean_dir / f"radial_{k}.bin")
+    np.arange(ny * nr, dtype=np.uint64).tofile(mean_dir / "radial_count.bin")
+    hdr = tmp_path / "constants.cuh"
+    hdr.write_text(
+        "// a header in the reference's declaration style\n"
+        "inline constexpr int NOUTPUT = 100;\n"
+        "inline constexpr label_t NX = static_cast<label_t>(6);\n"
+        "inline constexpr label_t NY = static_cast<label_t>(5); // streamwise\n"
+        "inline constexpr label_t NZ = static_cast<label_t>(6);\n"
+        "inline constexpr label_t NR_BINS = NX / static_cast<label_t>(2);\n"
+        "inline constexpr real_t jet_radius = static_cast<real_t>(1.5);\n"
+        "inline constexpr real_t jet_x0 = static_cast<real_t>(NX - 1) / static_cast<real_t>(2);\n"
+        "inline constexpr real_t jet_velocity = static_cast<real_t>(0.04);\n"
+        "inline constexpr real_t nu = (static_cast<real_t>(2) * jet_radius * jet_velocity) / 100.0;\n"
+        "inline constexpr std::array<int, 2> skipped = {1, 2};\n")
+    k = post.read_case_constants(hdr)
+    assert k["NX"] == 6 and k["NY"] == 5 and k["NR_BINS"] == 3 and k["NOUTPUT"] == 100
+    assert k["jet_x0"] == 2.5 and k["D"] == 3.0 and k["U_jet"] == pytest.approx(0.04)
+    assert k["nu"] == pytest.approx(2 * 1.5 * 0.04 / 100.0)
+    assert "skipped" not in k
+    with pytest.raises(ValueError):
+        post.read_radial_moments(mean_dir)
+    got = post.read_radial_moments(mean_dir, constants_file=hdr)
+    assert got["count"].shape == (ny, nr) and got["metadata"]["D"] == 3.0
+    # case_info.txt of the case folder takes precedence over the constants header
+    c = m.jet_case(6, 5, 6, 1.0)
+    mio.write_case_info(c, str(case_dir))
+    md = post.case_metadata(mean_dir, constants_file=hdr)
+    assert md["NY"] == 5 and md["D"] == 2.0 and md["U_jet"] == pytest.approx(0.05)

@@ -91,9 +91,9 @@ def test_perturbation_mass_leak_at_weak_gradients():
 def predicted_stale(case):
     """Interior slots the reference never writes after initialisation:
     Q1 - c_y=+1 slots of plane y=1 whose ghost source (x-cx, 0, z-cz) is not a
-    jet node of the inner frame (jet/boundary_conditions.cuh:102-105,137);
+    jet node of the inner frame (jet/boundary_conditions.cuh:38-41,73);
     Q2 - c_y=-1 slots of plane y=NY-2 whose ghost source lies on the x/z frame
-    (neumann runs on inner columns only and writes without wrap, :286)."""
+    (neumann runs on inner columns only and writes without wrap, :222)."""
     L, c = c_vectors()
     NX, NY, NZ = case.NX, case.NY, case.NZ
     stale = np.zeros((27, NZ, NY, NX), bool)
@@ -273,7 +273,7 @@ def set_uy_column(o, case, uy_target):
 
 @pytest.mark.parametrize("uy,uc", [(-0.3, 0.0), (0.4, 0.4), (1.6, 1.0)])
 def test_outlet_convective_clamp(uy, uc):
-    """phi_B <- phi_B - clamp01(u_y,F) (phi_B - phi_F) (jet/boundary_conditions.cuh:246-254)."""
+    """phi_B <- phi_B - clamp01(u_y,F) (phi_B - phi_F) (jet/boundary_conditions.cuh:87-93,184-190)."""
     case = outlet_case()
     o = Oracle(case)
     o.init_jet()
