@@ -135,10 +135,12 @@ __device__ __forceinline__ float A_node(const DevParams &P, float rr, float rb) 
 // FP16 / BF16: one 32-bit word per direction and node: the low half holds
 // t/w_i - 1 with t = f_r + f_b (the offset of the two-colour population from
 // its lattice weight, scaled by 1/w_i), the high half holds the minority
-// colour m/w_i with its sign bit naming the minority colour (+: blue is the
-// minority, -: red is).  Pure phases (m = 0) are therefore exact, and the
-// arithmetic stays FP32.  Populations are assumed non-negative (they are in a
-// stable run; the watchdog flags non-finite states).
+// colour m/w_i -- the colour of smaller magnitude, with its own sign -- whose
+// mantissa LSB is replaced by the colour flag (1: red is the minority, 0:
+// blue).  The decode clears the flag bit before converting, so a minority of
+// 0 stays exactly 0: pure phases are exact also where their populations are
+// negative, and a negative minority keeps its sign.  The minority keeps 9
+// (FP16) or 6 (BF16) mantissa bits; the arithmetic stays FP32.
 constexpr float invw(int i) { return 1.0f / w(i); }
 
 __device__ __forceinline__ uint32_t f32_bits(float v) { return __builtin_bit_cast(uint32_t, v); }
@@ -200,13 +202,13 @@ struct Store16 {
             return bits_f32(h << 16);
         }
     }
+    static constexpr uint32_t kFlag = 0x10000u;  // LSB of the high half: red is the minority
     template <int I>
     __device__ __forceinline__ static void decode(uint32_t v, float &fr, float &fb) {
         const float a = half_value(v & 0xffffu);
-        const uint32_t hb = v >> 16;
         const float t = (a + 1.0f) * w(I);
-        const float m = half_value(hb & 0x7fffu) * w(I);
-        if (hb & 0x8000u) {
+        const float m = half_value((v >> 16) & 0xfffeu) * w(I);
+        if (v & kFlag) {
             fr = m;
             fb = t - m;
         } else {
@@ -214,28 +216,33 @@ struct Store16 {
             fr = t - m;
         }
     }
-    // The word: low half a (round to nearest even), high half the minority
-    // magnitude with its sign bit set when red is the minority.  Built as the
-    // float pair (a, +-|m|) and converted in one packed instruction
-    // (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32): rounding is symmetric, so the
-    // sign of -|m| lands exactly on the flag bit.
-    __device__ __forceinline__ static uint32_t pack(float lo, float hi) {
-        typedef float f2 __attribute__((ext_vector_type(2)));
+    // the flag bit inserted into a word whose high half is already rounded
+    // to an even mantissa (v_bfi_b32)
+    __device__ __forceinline__ static uint32_t with_flag(uint32_t v, bool red_minor) {
+        return (v & ~kFlag) | (red_minor ? kFlag : 0u);
+    }
+    // The word from a = t/w - 1 and the signed minority m/w, both rounded to
+    // nearest: the low half by the convert (ties to even), the high half on
+    // the grid of its even mantissas (ties away from zero): half of that grid
+    // step (2^13 of the FP32 pattern for FP16, 2^16 for BF16) is added before
+    // the truncation drops 14 (FP16) or 17 (BF16) bits.
+    __device__ __forceinline__ static uint32_t pack_rn(float lo, float hi, bool red_minor) {
+        uint32_t l, h;
         if constexpr (ST == MCG_STORE_FP16) {
-            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-            return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{lo, hi}, h2));
+            l = __builtin_bit_cast(uint16_t, _Float16(lo));
+            const uint32_t hb = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(bits_f32(f32_bits(hi) + 0x2000u), 0.0f));
+            h = hb & 0xfffeu;
         } else {
-            typedef __bf16 h2 __attribute__((ext_vector_type(2)));
-            return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{lo, hi}, h2));
+            l = __builtin_bit_cast(uint16_t, __bf16(lo));
+            h = ((f32_bits(hi) + 0x10000u) >> 16) & 0xfffeu;
         }
+        return l | (h << 16) | (red_minor ? kFlag : 0u);
     }
     template <int I>
     __device__ __forceinline__ static uint32_t encode(float fr, float fb) {
-        const float t = fr + fb;
-        const float a = t * invw(I) - 1.0f;
-        const bool red_major = fr >= fb;
-        const float m = red_major ? fabsf(fb) * invw(I) : -(fabsf(fr) * invw(I));
-        return pack(a, m);
+        const float a = (fr + fb) * invw(I) - 1.0f;
+        const bool red_minor = fabsf(fr) < fabsf(fb);
+        return pack_rn(a, (red_minor ? fr : fb) * invw(I), red_minor);
     }
     template <int I>
     __device__ __forceinline__ static void load(const void *f, uint32_t stride, uint32_t off, float &fr, float &fb) {
@@ -255,40 +262,41 @@ struct Store16 {
         __builtin_nontemporal_store(encode<I>(fr, fb), at(f, uint32_t(I) * stride + off));
     }
     // The same word with stochastic rounding: a uniform random integer below
-    // the dropped mantissa bits (13 for FP16, 16 for BF16) is added to the
-    // FP32 bit pattern, and the result is truncated toward zero.  The pattern
-    // is sign-magnitude and ordered by magnitude, so the magnitude rounds up
-    // with probability equal to the dropped fraction and is truncated
-    // otherwise: E[stored] = value, and the sign is kept (also for -0), so the
-    // colour flag survives.  A steady flow then carries zero-mean storage
-    // noise instead of the same rounding error every step.  FP16: one packed
-    // round-toward-zero convert (v_cvt_pkrtz_f16_f32); below the FP16 normal
-    // range (|x| < 2^-14) more than 13 bits drop and the rounding leans toward
-    // zero by less than 2^-24.  BF16: the two high halves are the word (one
-    // byte permute).  Integer adds are used because the v_cvt_sr_* converts
-    // ran the FP16 step 5% slower.
-    __device__ __forceinline__ static uint32_t pack_sr(float lo, float hi, uint32_t r) {
+    // the dropped bits is added to the FP32 bit pattern, and the result is
+    // truncated toward zero.  The pattern is sign-magnitude and ordered by
+    // magnitude, so the magnitude rounds up with probability equal to the
+    // dropped fraction and is truncated otherwise: E[stored] = value, and the
+    // sign is kept (also for -0).  The low half drops 13 (FP16) or 16 (BF16)
+    // bits, the high half one more, the bit the flag then takes.  A steady
+    // flow thus carries zero-mean storage noise instead of the same rounding
+    // error every step.  FP16: one packed round-toward-zero convert
+    // (v_cvt_pkrtz_f16_f32) of both halves; below the FP16 normal range
+    // (|x| < 2^-14) more bits drop and the rounding leans toward zero by less
+    // than 2^-24.  BF16: the two high halves are the word (one byte permute).
+    // Integer adds are used because the v_cvt_sr_* converts ran the FP16 step
+    // 5% slower.
+    __device__ __forceinline__ static uint32_t pack_sr(float lo, float hi, uint32_t r, bool red_minor) {
+        uint32_t v;
         if constexpr (ST == MCG_STORE_FP16) {
-            const uint32_t d = r >> 19;
-            return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(bits_f32(f32_bits(lo) + d),
-                                                                           bits_f32(f32_bits(hi) + d)));
+            v = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(bits_f32(f32_bits(lo) + (r >> 19)),
+                                                                        bits_f32(f32_bits(hi) + (r >> 18))));
         } else {
-            // d = r >> 16 as a byte permute (selector 0x0c: a zero byte): the
+            // r >> 16 as a byte permute (selector 0x0c: a zero byte): the
             // compiler would otherwise fold the shift into word-select (SDWA)
             // adds, which ran the BF16 step 4.5% slower
             const uint32_t d = __builtin_amdgcn_perm(0u, r, 0x0c0c0302u);
-            // bytes 2,3 of (lo + d), then bytes 2,3 of (hi + d)
-            return __builtin_amdgcn_perm(f32_bits(hi) + d, f32_bits(lo) + d, 0x07060302u);
+            // bytes 2,3 of (lo + d), then bytes 2,3 of (hi + (r >> 15))
+            v = __builtin_amdgcn_perm(f32_bits(hi) + (r >> 15), f32_bits(lo) + d, 0x07060302u);
         }
+        return with_flag(v, red_minor);
     }
     // word from the offset a = t/w - 1 and the colours already divided by w,
     // stochastically rounded with the random word r (both halves)
     template <int I>
     __device__ __forceinline__ static void store_w(void *f, uint32_t stride, uint32_t off, float a, float rw, float bw,
                                                    uint32_t r) {
-        const bool red_major = rw >= bw;
-        __builtin_nontemporal_store(pack_sr(a, red_major ? fabsf(bw) : -fabsf(rw), r),
-                                    at(f, uint32_t(I) * stride + off));
+        const bool red_minor = fabsf(rw) < fabsf(bw);
+        __builtin_nontemporal_store(pack_sr(a, red_minor ? rw : bw, r, red_minor), at(f, uint32_t(I) * stride + off));
     }
 };
 template <>

@@ -172,6 +172,7 @@ struct OracleBase {
     virtual void gradient(int x, int y, int z, double F[3]) = 0;
     virtual int64_t get_step() = 0;
     virtual void set_inlet_offset(const double *du) = 0;
+    virtual double uy_dot(const double *w) = 0;
 };
 
 template <class T>
@@ -643,6 +644,14 @@ struct Oracle final : OracleBase {
             z[i] = double(uz[i]);
         }
     }
+    // sum_x w(x) u_y(x) over the grid, u_y as get_fields returns it (the
+    // moments pass of the last step); nodes with w = 0 are skipped
+    double uy_dot(const double *w) override {
+        double acc = 0.0;
+        for (size_t i = 0; i < N; ++i)
+            if (w[i] != 0.0) acc += w[i] * double(uy[i]);
+        return acc;
+    }
     void get_moments(double *out) override {
         const std::vector<T> *arr[11] = {&rhor, &rhob, &ux, &uy, &uz, &Pxx, &Pxy, &Pyy, &Pyz, &Pzz, &Pxz};
         for (int k = 0; k < 11; ++k)
@@ -767,6 +776,7 @@ void mcgo_global_sums(void *h, double *o) { static_cast<OracleBase *>(h)->global
 int mcgo_writeset(void *h, uint8_t *cr, uint8_t *cb) { return static_cast<OracleBase *>(h)->writeset(cr, cb); }
 void mcgo_gradient(void *h, int x, int y, int z, double *F) { static_cast<OracleBase *>(h)->gradient(x, y, z, F); }
 void mcgo_set_inlet_offset(void *h, const double *du) { static_cast<OracleBase *>(h)->set_inlet_offset(du); }
+double mcgo_uy_dot(void *h, const double *w) { return static_cast<OracleBase *>(h)->uy_dot(w); }
 
 // ---- single-node operators, exposed for the invariant tests (FP64) ----
 static const Lattice<double> &lat() {

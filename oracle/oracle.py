@@ -40,6 +40,7 @@ _SIGS = [
     ("mcgo_writeset", c_int, [c_void_p, POINTER(c_uint8), POINTER(c_uint8)]),
     ("mcgo_gradient", None, [c_void_p, c_int, c_int, c_int, _D]),
     ("mcgo_set_inlet_offset", None, [c_void_p, _D]),
+    ("mcgo_uy_dot", c_double, [c_void_p, _D]),
     ("mcgo_lattice", None, [POINTER(c_int), POINTER(c_int), POINTER(c_int), _D, _D, _D]),
     ("mcgo_feq", c_double, [c_int, c_double, c_double, c_double, c_double]),
     ("mcgo_fneqr", c_double, [c_int, _D, c_double, c_double, c_double]),
@@ -210,6 +211,12 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("writeset failed")
         return cr, cb
+
+    def uy_dot(self, w):
+        """sum_x w(x) u_y(x) over the grid (w shaped like the fields), u_y of the
+        moments pass of the last step: a u_y functional without copying the fields."""
+        w = np.ascontiguousarray(w, np.float64).reshape(self.shape)
+        return lib().mcgo_uy_dot(self._h, _p(w))
 
     def gradient(self, x, y, z):
         F = np.zeros(3)

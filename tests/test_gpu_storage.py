@@ -1,10 +1,11 @@
 """16-bit population storage (FP16 / BF16 offsets, FP32 arithmetic).
 
 Encoding per (node, direction): t/w_i - 1 with t = f_r + f_b, and the
-minority colour m/w_i whose sign bit names the minority colour.
-Round-trip bounds (round to nearest, |t/w - 1| < 1/4 here):
-  FP16: |d t| <= 2^-14 w_i (1 + |t|/w_i 2^-2),  |d m| <= 2^-11 m
-  BF16: |d t| <= 2^-11 w_i,                      |d m| <= 2^-8 m
+minority colour m/w_i (the colour of smaller magnitude, signed) whose mantissa
+LSB is the colour flag.  Round-trip bounds (round to nearest; a = t/w_i - 1 is
+stored with p = 10 (FP16) or 7 (BF16) mantissa bits, the minority with p - 1):
+  FP16: |d t| <= 2^-11 |a| w_i,  |d m| <= 2^-10 |m|  (+ 2^-23 w_i below the normal range)
+  BF16: |d t| <= 2^-8 |a| w_i,   |d m| <= 2^-7 |m|
 """
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ from conftest import INNER
 
 pytestmark = pytest.mark.gpu
 
-BOUNDS = {"fp16": (2.0 ** -14 * 1.3, 2.0 ** -11), "bf16": (2.0 ** -11, 2.0 ** -8)}
+BOUNDS = {"fp16": (2.0 ** -11, 2.0 ** -10, 2.0 ** -23), "bf16": (2.0 ** -8, 2.0 ** -7, 0.0)}
 
 
 def weights():
@@ -24,31 +25,52 @@ def weights():
 
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
 def test_round_trip_bounds_and_pure_phases(mcg, storage):
-    case = mcg.jet_case(20, 16, 18, jet_radius=3.0)
+    """x slabs of the interior: 1-5 pure red, 6-8 pure blue, 9-10 pure red and
+    11-12 pure blue with negative populations, 13-14 a negative red minority,
+    15-16 a negative blue minority next to a positive majority, the rest mixed."""
+    case = mcg.jet_case(24, 16, 18, jet_radius=3.0)
     s = mcg.Solver(case, storage=storage)
     s.init_jet()
     rng = np.random.default_rng(3)
     w = weights()
     shape = s.shape
-    frac = rng.random(shape)
-    frac[:, :, :6] = 0.0  # pure red
-    frac[:, :, 6:9] = 1.0  # pure blue
-    t = w * (1 + 0.2 * (rng.random((27,) + shape) - 0.5))
-    fr = (t * (1 - frac)).astype(np.float32)
-    fb = (t * frac).astype(np.float32)
+    pops = (27,) + shape
+    frac = rng.random(pops)
+    frac[..., :6] = 0.0  # pure red
+    frac[..., 6:9] = 1.0  # pure blue
+    t = w * (1 + 0.2 * (rng.random(pops) - 0.5))
+    fr = t * (1 - frac)
+    fb = t * frac
+    neg = -w * (0.01 + 0.2 * rng.random(pops))  # negative populations
+    fr[..., 9:11], fb[..., 9:11] = neg[..., 9:11], 0.0
+    fr[..., 11:13], fb[..., 11:13] = 0.0, neg[..., 11:13]
+    fr[..., 13:15] = 0.3 * neg[..., 13:15]  # small negative red next to positive blue
+    fb[..., 15:17] = 0.3 * neg[..., 15:17]  # small negative blue next to positive red
+    fr, fb = fr.astype(np.float32), fb.astype(np.float32)
     s.set_populations(fr, fb)
     gr, gb = s.populations()
     sl = (slice(None),) + INNER
-    bt, bm = BOUNDS[storage]
+    bt, bm, floor = BOUNDS[storage]
     tt = (fr + fb).astype(np.float64)
-    mm = np.minimum(fr, fb).astype(np.float64)
+    ww = w * np.ones(shape)
+    a = tt / ww - 1.0
+    # the FP32 sum fr + fb and the decode's (a + 1) w add a few 1e-7 of |t|
     dt = np.abs((gr + gb).astype(np.float64) - tt)[sl]
-    assert np.all(dt <= bt * (w * np.ones(shape))[sl] + 2e-7 * tt[sl])
-    dm = np.abs(np.minimum(gr, gb).astype(np.float64) - mm)[sl]
-    assert np.all(dm <= bm * mm[sl] + 1e-12)
-    # pure phases stay exactly pure
-    assert np.all(gb[(slice(None),) + (slice(1, -1), slice(1, -1), slice(1, 6))] == 0)
-    assert np.all(gr[(slice(None),) + (slice(1, -1), slice(1, -1), slice(6, 9))] == 0)
+    assert np.all(dt <= (bt * np.abs(a) * ww + 3e-7 * np.abs(tt) + floor * ww)[sl])
+    red_minor = np.abs(fr) < np.abs(fb)
+    mm = np.where(red_minor, fr, fb).astype(np.float64)
+    gm = np.where(red_minor, gr, gb).astype(np.float64)
+    dm = np.abs(gm - mm)[sl]
+    assert np.all(dm <= (bm * np.abs(mm) + floor * ww)[sl] + 1e-12)
+    # signs survive, including negative minorities and negative pure phases
+    nz = (mm != 0)[sl]
+    assert np.all(np.sign(gm[sl][nz]) == np.sign(mm[sl][nz]))
+    assert np.any(mm[sl] < 0) and np.any((tt[sl] < 0) & ~nz)
+    # pure phases stay exactly pure, whatever the sign of the majority
+    inner_x = (slice(None), slice(1, -1), slice(1, -1))
+    for x0, x1, zero in ((1, 6, gb), (6, 9, gr), (9, 11, gb), (11, 13, gr)):
+        assert np.all(zero[inner_x + (slice(x0, x1),)] == 0), (x0, x1)
+    assert np.all(gr[inner_x + (slice(9, 11),)] < 0) and np.all(gb[inner_x + (slice(11, 13),)] < 0)
 
 
 @pytest.mark.parametrize("storage,tol_u", [("fp16", 2e-4), ("bf16", 2e-3)])

@@ -132,9 +132,9 @@ def test_gpu_sponge_profile(mcg):
 @pytest.mark.parametrize("kernel", ["fused", "twopass"])
 def test_gpu_jet_with_sponge_matches_oracle(mcg, kernel):
     """Reference-semantics jet (sharp inlet, reference gradient) with a sponge
-    of 10 planes in front of the outlet, K = 4, against the FP64 oracle."""
+    of 10 planes in front of the outlet, K = 20, against the FP64 oracle."""
     c = mcg.jet_case(24, 28, 24, jet_radius=4.0, jet_velocity=0.05, Re=300.0)
-    c.sponge_K, c.sponge_cells = 4.0, 10
+    c.sponge_K, c.sponge_cells = 20.0, 10
     s = mcg.Solver(c, kernel=kernel)
     o = Oracle(c)
     s.init_jet()
@@ -147,7 +147,8 @@ def test_gpu_jet_with_sponge_matches_oracle(mcg, kernel):
             tol = (5e-6 if k.startswith("rho") else 1e-6) * scale
             e = float(np.max(np.abs(g[k].astype(np.float64) - r[k])))
             assert e <= tol, f"{kernel} {k} after {steps}: {e:.3e}"
-    # the sponge is active: a run without it differs far beyond these bounds
+    # the sponge is active: without it u_y differs by 2.5e-4 (rho_r 5.9e-4),
+    # far beyond the bounds above
     c0 = mcg.jet_case(24, 28, 24, jet_radius=4.0, jet_velocity=0.05, Re=300.0)
     o0 = Oracle(c0)
     o0.init_jet()
