@@ -207,6 +207,18 @@ int mcg_tke(mcg_solver *s, double *tke_total);
  * out[2..4]=sum rho*u.  Extension (the reference has none). */
 int mcg_global_sums(mcg_solver *s, double out[5]);
 
+/* The same FP64 sums as mcg_tke, mcg_global_sums and the watchdog, written
+ * to device memory on the solver's stream without allocating anything and
+ * without synchronising: out_dev (7 doubles of device memory) receives
+ * {sum 0.5|u|^2, sum rho_r, sum rho_b, sum rho u_x, sum rho u_y, sum rho u_z,
+ * number of interior nodes the watchdog rejects}.  The partial sums live in a
+ * library-owned device array of 8192 doubles per GPU (64 KiB, held until the
+ * process ends); a solver needing more partials is rejected with
+ * MCG_E_INVALID.  Calls that overlap on one GPU race on that array.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). */
+int mcg_global_sums_noalloc(mcg_solver *s, double *out_dev);
+
 /* Watchdog: returns MCG_E_NUMERIC if any interior density is NaN/Inf or <= 0. */
 int mcg_check_health(mcg_solver *s);
 

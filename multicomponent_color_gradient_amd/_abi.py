@@ -132,6 +132,7 @@ SIGNATURES = [
     ("mcg_copy_radial", c_int, [_PSOLVER, _D, _D, _D, _D, _D, POINTER(c_uint64), _D, _D]),
     ("mcg_tke", c_int, [_PSOLVER, _D]),
     ("mcg_global_sums", c_int, [_PSOLVER, _D]),
+    ("mcg_global_sums_noalloc", c_int, [_PSOLVER, c_void_p]),
     ("mcg_check_health", c_int, [_PSOLVER]),
     ("mcg_copy_fields", c_int, [_PSOLVER, _F, _F, _F, _F, _F]),
     ("mcg_copy_populations", c_int, [_PSOLVER, _F, _F]),

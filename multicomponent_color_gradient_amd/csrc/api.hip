@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <functional>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mcg.h"
@@ -52,6 +53,33 @@ constexpr int kInitJet = 0, kInitBubble = 1, kInitBubbleTanh = 2, kInitSpheres =
             default: hipLaunchKernelGGL((KERNEL<MCG_STORE_FP32>), __VA_ARGS__); break;   \
         }                                                                                \
     } while (0)
+
+// Partial sums of mcg_global_sums_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends.  Its address is resolved once per
+// device (in mcg_create, with that device current) and kept in
+// g_scratch_ptr[device]; host code passes that pointer, never the array's
+// name, which on the host is only a stand-in.  Every call on a GPU shares the
+// array: calls that overlap on one GPU race.  Each call overwrites every
+// partial it reads, so nothing needs clearing between calls.
+constexpr size_t kScratchDoubles = 8192;  // 64 KiB: kRedQ partials of up to 1170 workgroups
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+double *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// resolve this device's copy of the scratch array (the current device)
+int scratch_resolve(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(MCG_E_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MCG_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(MCG_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double *>(ptr);
+    return MCG_OK;
+}
 
 }  // namespace
 
@@ -514,6 +542,10 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
     }
     if (hipDeviceGetAttribute(&s->num_cu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || s->num_cu <= 0)
         s->num_cu = 256;
+    if (int rc = scratch_resolve(s->device)) {
+        delete s;
+        return rc;
+    }
     {
         // histograms larger than the default 64 KiB need the kernel's dynamic-LDS limit raised
         int lds = 0;
@@ -694,6 +726,25 @@ int mcg_global_sums(mcg_solver *s, double out[5]) {
     double r[kRedQ];
     if (int rc = run_reduce(s, r)) return rc;
     for (int k = 0; k < 5; ++k) out[k] = r[1 + k];
+    return MCG_OK;
+}
+
+int mcg_global_sums_noalloc(mcg_solver *s, double *out_dev) {
+    if (int rc = check_ready(s)) return rc;
+    if (!out_dev) return fail(MCG_E_INVALID, "null output pointer");
+    const size_t need = size_t(kRedQ) * size_t(s->red_blocks);
+    if (need > kScratchDoubles)
+        return fail(MCG_E_INVALID, "global sums: " + std::to_string(need) + " partial sums exceed the scratch array of " +
+                                       std::to_string(kScratchDoubles) + " doubles");
+    double *part = (s->device >= 0 && s->device < kMaxDevices) ? g_scratch_ptr[s->device] : nullptr;
+    if (!part) return fail(MCG_E_STATE, "scratch array of this device was not resolved");
+    if (int rc = ensure_moments(s)) return rc;
+    const size_t total = size_t(s->c.NX - 2) * (s->c.NY - 2) * (s->c.NZ - 2);
+    const int npb = int((total + s->red_blocks - 1) / s->red_blocks);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(s->red_blocks), dim3(256), 0, s->stream, s->P, s->mom, part, npb);
+    LAUNCH_CHECK("k_reduce_partials");
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s->stream, part, s->red_blocks, out_dev);
+    LAUNCH_CHECK("k_reduce_final");
     return MCG_OK;
 }
 
