@@ -90,6 +90,40 @@ def copy_probe(nbytes: int, reps: int = 3) -> None:
         hip.hipFree(b)
 
 
+PROBE_KINDS = {"read": 0, "write": 1, "copy": 2}
+
+
+def hbm_ceiling(full: bool, nbytes: int = 4 << 30, reps: int = 10) -> dict:
+    """The bandwidth plain streams reach on this device (mcg_hbm_probe): read-only,
+    write-only and copy over ``nbytes`` buffers, for each access width (4, 8, 16
+    bytes per lane), cached or non-temporal stores and workgroups per CU; the
+    best of ``reps`` launches per variant.  ``full`` sweeps every variant, else
+    only 16-byte accesses with 8 workgroups per CU."""
+    import ctypes
+
+    import multicomponent_color_gradient_amd as m
+
+    lib = m.load_library()
+    rows = []
+    widths = (4, 8, 16) if full else (16,)
+    per_cu = (2, 4, 8, 16) if full else (8,)
+    for kind, k in PROBE_KINDS.items():
+        for vb in widths:
+            for nt in ((0, 1) if kind != "read" else (0,)):
+                for bpc in per_cu:
+                    g = ctypes.c_double()
+                    rc = lib.mcg_hbm_probe(k, vb, nt, bpc, ctypes.c_uint64(nbytes), reps, ctypes.byref(g))
+                    if rc != 0:
+                        raise RuntimeError(f"hbm probe failed: {lib.mcg_last_error().decode()}")
+                    rows.append({"kind": kind, "vec_bytes": vb, "nontemporal": nt, "blocks_per_cu": bpc,
+                                 "gbs": round(g.value, 1)})
+                    print(f"hbm probe: {kind} {vb} B/lane nt={nt} {bpc} wg/CU: {g.value:.1f} GB/s",
+                          file=sys.stderr, flush=True)
+    best = {kind: max((r for r in rows if r["kind"] == kind), key=lambda r: r["gbs"]) for kind in PROBE_KINDS}
+    return {"buffer_bytes": nbytes, "reps": reps, "best_gbs": {k: v["gbs"] for k, v in best.items()},
+            "best": best, "rows": rows}
+
+
 def adjoint_ratio(steps: int, every: int) -> dict:
     """Adjoint/forward step-cost ratio on the 192x384x192 rising bubble (walls in
     x/z, periodic y, R=16 at y=64, gravity 1e-5 on red, tau 0.55/1.0, A=1e-3,
@@ -251,6 +285,9 @@ def main():
     ap.add_argument("--no-interface-rich", action="store_true", help="skip the interface-rich (256 spheres) run")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=200)
+    ap.add_argument("--ceiling", action="store_true",
+                    help="only measure the HBM bandwidth of plain read, write and copy streams (every probe "
+                         "variant) and print it as one JSON line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a device buffer of this size 3 times with hipMemcpy "
                          "(a plain copy kernel of known size that calibrates rocprofv3's FETCH_SIZE/WRITE_SIZE)")
@@ -262,6 +299,10 @@ def main():
         sys.exit(2)
 
     import multicomponent_color_gradient_amd as m
+
+    if args.ceiling:
+        print(json.dumps({"hbm_ceiling": hbm_ceiling(full=True)}), flush=True)
+        return
 
     case, init = make_case(args.workload)
 
@@ -280,6 +321,7 @@ def main():
         wall = time.perf_counter() - t0
         ms = sum(wms)
         s.check_health()
+        launch = s.launch_info()
         s.close()
         cells = case.NX * case.NY * case.NZ
         interior = m.interior_nodes(case)
@@ -290,7 +332,8 @@ def main():
         return {"mlups": args.steps * cells / (ms / 1e3) / 1e6, "mlups_int": args.steps * interior / (ms / 1e3) / 1e6,
                 "avg_ms": avg_ms, "wall": wall, "bpl": bpl, "achieved": achieved,
                 "traffic": load_traffic(args.workload, storage),
-                "windows": [round(x, 5) for x in per_window], "median_ms": float(sorted(per_window)[len(per_window) // 2])}
+                "windows": [round(x, 5) for x in per_window], "median_ms": float(sorted(per_window)[len(per_window) // 2]),
+                "launch": launch}
 
     r = measure(args.storage)
     mlups, mlups_int, avg_ms, wall = r["mlups"], r["mlups_int"], r["avg_ms"], r["wall"]
@@ -322,7 +365,8 @@ def main():
                    "NX": case.NX, "NY": case.NY, "NZ": case.NZ, "storage": args.storage, "kernel": args.kernel,
                    "stats_every": case.NSTATS_SAMPLE, "mlups_interior": round(mlups_int, 2),
                    "wall_s": round(wall, 4), "windows_ms_per_step": r["windows"],
-                   "mlups_median_window": round(case.NX * case.NY * case.NZ / r["median_ms"] / 1e3, 2)},
+                   "mlups_median_window": round(case.NX * case.NY * case.NZ / r["median_ms"] / 1e3, 2),
+                   "launch": r["launch"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_lu": bpl, "kernel_ms": round(avg_ms, 5)},
@@ -332,6 +376,7 @@ def main():
             continue
         out[key] = {"value": round(rr["mlups"], 2), "unit": "MLUPS", "ms_per_step": round(rr["avg_ms"], 5),
                     "speedup_vs_fp32": round(rr["mlups"] / mlups, 3), "windows_ms_per_step": rr["windows"],
+                    "launch": rr["launch"],
                     "mlups_median_window": round(case.NX * case.NY * case.NZ / rr["median_ms"] / 1e3, 2),
                     "roofline": {"achieved": round(rr["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": round(rr["achieved"] / HBM_PEAK_GBS, 4), "traffic": rr["traffic"],

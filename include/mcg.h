@@ -141,6 +141,19 @@ int mcg_destroy(mcg_solver *s);
 /* Select the step kernel (MCG_KERNEL_*).  Both give identical results. */
 int mcg_set_kernel(mcg_solver *s, int kernel);
 
+/* Launch shape of the fused step (an extension; the reference launches fixed
+ * block shapes, utilities/cudaConfig.cuh:22-31).  The step kernel cuts z into
+ * segments; by default the solver's first plain steps (no statistics) try the
+ * candidate segment counts in turn, each timed with HIP events and waited for,
+ * and keep the fastest of two passes.  The results do not depend on the
+ * segmentation.  enable = 0 uses a fixed rule instead and never waits; 1
+ * (re)starts the timing. */
+int mcg_set_launch_tuning(mcg_solver *s, int enable);
+
+/* The current launch shape: z segments, workgroups per step, and whether the
+ * timing has chosen it.  Any pointer may be NULL. */
+int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned);
+
 /* ---- case API: initial conditions ------------------------------------- */
 
 /* Jet initial state: launch_jetDensity + launch_Injet (launch.cuh:19-29,
@@ -371,6 +384,17 @@ int mcg_cpu_adjoint_jet(const mcg_case *c, int nsteps, const double *fr0, const 
 /* Run nsteps steps bracketed by HIP events on the solver's stream and return
  * the elapsed milliseconds (pure step launches, no host sync inside). */
 int mcg_time_steps(mcg_solver *s, int nsteps, double *ms);
+
+/* The device's achievable HBM bandwidth for a plain stream over nbytes of
+ * device memory (allocated and freed by the call, on the current device and
+ * the null stream): kind 0 reads, 1 writes, 2 copies to a second buffer; each
+ * lane moves vec_bytes (4, 8 or 16) per access with 4 accesses in flight, over
+ * blocks_per_cu x (number of CUs) workgroups of 256 threads; nontemporal != 0
+ * streams the stores past L2.  *gbs = the best of reps timed launches after a
+ * warm-up, in GB/s of bytes moved (copies count read + write).  The ceiling
+ * the step kernels' traffic is compared with.  No counterpart in the
+ * reference (an extension). */
+int mcg_hbm_probe(int kind, int vec_bytes, int nontemporal, int blocks_per_cu, uint64_t nbytes, int reps, double *gbs);
 
 /* Bytes of device memory held by the solver. */
 int mcg_device_bytes(const mcg_solver *s, uint64_t *nbytes);

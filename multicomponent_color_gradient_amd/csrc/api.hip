@@ -17,6 +17,7 @@
 #include "device.hpp"
 #include "fused.hpp"
 #include "kernels.hpp"
+#include "probe.hpp"
 
 using namespace mcg;
 
@@ -106,6 +107,16 @@ struct mcg_solver {
     int red_blocks = 0;
     float *staging = nullptr;
     size_t staging_bytes = 0;
+
+    // launch shape of the fused step: the static choice, then the fastest of the
+    // candidate z segmentations timed on the first plain steps (two passes)
+    FusedGrid grid{};
+    std::vector<FusedGrid> tune_cand;
+    std::vector<float> tune_ms;
+    int tune_next = 0;
+    bool tune_on = true, tuned = false;
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};
+    static constexpr int kTunePasses = 2;
 
     int cur = 0;
     int64_t step = 0;
@@ -368,6 +379,38 @@ int do_init(mcg_solver *s, int kind) {
     return MCG_OK;
 }
 
+// One fused step.  Until the launch shape is tuned, plain steps (no
+// statistics, no recorded moments) run the candidate segmentations in turn,
+// each between two events and waited for, and the fastest of two passes is
+// kept; every other step uses the current choice.
+int launch_step(mcg_solver *s, const DevBufs &B) {
+    const bool tuning = s->tune_on && !s->tuned && !B.do_stats && !B.mom_out && s->tune_cand.size() > 1;
+    if (!tuning) {
+        const int rc = launch_fused(s->P, B, s->stream, s->grid);
+        if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
+        return MCG_OK;
+    }
+    for (hipEvent_t &e : s->tune_ev)
+        if (!e) HIP_TRY(hipEventCreate(&e));
+    const size_t n = s->tune_cand.size();
+    const size_t k = size_t(s->tune_next) % n;
+    HIP_TRY(hipEventRecord(s->tune_ev[0], s->stream));
+    const int rc = launch_fused(s->P, B, s->stream, s->tune_cand[k]);
+    if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
+    HIP_TRY(hipEventRecord(s->tune_ev[1], s->stream));
+    HIP_TRY(hipEventSynchronize(s->tune_ev[1]));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, s->tune_ev[0], s->tune_ev[1]));
+    if (s->tune_ms.size() != n) s->tune_ms.assign(n, 1e30f);
+    s->tune_ms[k] = std::min(s->tune_ms[k], ms);
+    if (++s->tune_next >= int(n) * mcg_solver::kTunePasses) {
+        const size_t best = size_t(std::min_element(s->tune_ms.begin(), s->tune_ms.end()) - s->tune_ms.begin());
+        s->grid = s->tune_cand[best];
+        s->tuned = true;
+    }
+    return MCG_OK;
+}
+
 int one_step(mcg_solver *s) {
     // two-pass kernel: the moments pass itself writes the recorded moments
     struct MomSwap {
@@ -421,8 +464,7 @@ int one_step(mcg_solver *s) {
             Bf.mom_out = s->mom;  // the step stores the moments of its source state
             have_moments = true;
         }
-        const int rc = launch_fused(s->P, Bf, s->stream, s->num_cu);
-        if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
+        if (int rc = launch_step(s, Bf)) return rc;
         s->moments_valid = have_moments;
     }
     if (observe) {
@@ -546,6 +588,8 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
         delete s;
         return rc;
     }
+    s->grid = fused_grid(s->P, s->num_cu);
+    s->tune_cand = fused_candidates(s->P);
     {
         // histograms larger than the default 64 KiB need the kernel's dynamic-LDS limit raised
         int lds = 0;
@@ -599,6 +643,8 @@ int mcg_destroy(mcg_solver *s) {
     for (void *p : {(void *)s->grr, (void *)s->grb, (void *)s->mom, (void *)s->stats, (void *)s->count,
                     (void *)s->red_partials, (void *)s->red_out})
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : s->tune_ev)
+        if (e) (void)hipEventDestroy(e);
     s->release_staging();
     s->adj_release();
     s->uy_release();
@@ -611,6 +657,24 @@ int mcg_set_kernel(mcg_solver *s, int kernel) {
     if (int rc = check_solver(s)) return rc;
     if (kernel != MCG_KERNEL_FUSED && kernel != MCG_KERNEL_TWOPASS) return fail(MCG_E_INVALID, "unknown kernel");
     s->kernel = kernel;
+    return MCG_OK;
+}
+
+int mcg_set_launch_tuning(mcg_solver *s, int enable) {
+    if (int rc = check_solver(s)) return rc;
+    s->tune_on = enable != 0;
+    s->tuned = false;
+    s->tune_next = 0;
+    s->tune_ms.clear();
+    s->grid = fused_grid(s->P, s->num_cu);
+    return MCG_OK;
+}
+
+int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned) {
+    if (int rc = check_solver(s)) return rc;
+    if (segments) *segments = s->grid.nzs;
+    if (workgroups) *workgroups = s->grid.nwg;
+    if (tuned) *tuned = s->tuned ? 1 : 0;
     return MCG_OK;
 }
 
@@ -957,6 +1021,61 @@ int mcg_write_radial(mcg_solver *s, const char *mean_dir) {
                                           s->started ? 1 : 0, s->start_step, s->samples);
     if (rc) return fail(rc, mcg_io_last_error());
     return MCG_OK;
+}
+
+int mcg_hbm_probe(int kind, int vec_bytes, int nontemporal, int blocks_per_cu, uint64_t nbytes, int reps, double *gbs) {
+    if (!gbs || reps < 1 || blocks_per_cu < 1 || nbytes < (uint64_t(1) << 20) || (nbytes & 255) ||
+        (kind != kProbeRead && kind != kProbeWrite && kind != kProbeCopy) ||
+        (vec_bytes != 4 && vec_bytes != 8 && vec_bytes != 16))
+        return fail(MCG_E_INVALID, "hbm probe: kind 0..2, vec_bytes 4/8/16, blocks_per_cu >= 1, reps >= 1, "
+                                   "nbytes >= 1 MiB and a multiple of 256");
+    int dev = 0, ncu = 256;
+    HIP_TRY(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+    void *a = nullptr, *b = nullptr, *sink = nullptr;
+    if (hipMalloc(&a, nbytes) != hipSuccess || (kind == kProbeCopy && hipMalloc(&b, nbytes) != hipSuccess) ||
+        hipMalloc(&sink, 256) != hipSuccess) {
+        for (void *p : {a, b, sink})
+            if (p) (void)hipFree(p);
+        return fail(MCG_E_NOMEM, "hbm probe: device allocation failed");
+    }
+    const uint32_t *src = static_cast<const uint32_t *>(a);
+    uint32_t *dst = static_cast<uint32_t *>(kind == kProbeCopy ? b : a);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MCG_OK;
+    float best = 0.0f;
+    auto run = [&]() {
+        switch (kind) {
+            case kProbeRead: launch_probe_vw<kProbeRead>(vec_bytes, nontemporal != 0, blocks_per_cu * ncu, src, dst, nbytes,
+                                                         static_cast<uint32_t *>(sink)); break;
+            case kProbeWrite: launch_probe_vw<kProbeWrite>(vec_bytes, nontemporal != 0, blocks_per_cu * ncu, src, dst, nbytes,
+                                                           static_cast<uint32_t *>(sink)); break;
+            default: launch_probe_vw<kProbeCopy>(vec_bytes, nontemporal != 0, blocks_per_cu * ncu, src, dst, nbytes,
+                                                 static_cast<uint32_t *>(sink)); break;
+        }
+    };
+    if (hipMemset(a, 0x5A, nbytes) != hipSuccess || (b && hipMemset(b, 0, nbytes) != hipSuccess) ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(MCG_E_HIP, "hbm probe: set-up failed");
+    } else {
+        run();  // warm-up
+        for (int r = 0; r < reps && rc == MCG_OK; ++r) {
+            float ms = 0.0f;
+            if (hipEventRecord(e0, nullptr) != hipSuccess) rc = fail(MCG_E_HIP, "hbm probe: event record failed");
+            run();
+            if (rc == MCG_OK && (hipGetLastError() != hipSuccess || hipEventRecord(e1, nullptr) != hipSuccess ||
+                                 hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = fail(MCG_E_HIP, "hbm probe: launch or timing failed");
+            const double moved = double(nbytes) * (kind == kProbeCopy ? 2.0 : 1.0);
+            if (rc == MCG_OK && ms > 0.0f) best = std::max(best, float(moved / (double(ms) * 1e-3) / 1e9));
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (void *p : {a, b, sink})
+        if (p) (void)hipFree(p);
+    if (rc == MCG_OK) *gbs = double(best);
+    return rc;
 }
 
 int mcg_device_bytes(const mcg_solver *s, uint64_t *nbytes) {

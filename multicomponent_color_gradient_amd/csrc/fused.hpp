@@ -21,6 +21,7 @@
 #pragma once
 
 #include <algorithm>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -77,7 +78,8 @@ __device__ __forceinline__ FastSrc make_src(const DevParams &P, int x, int y, in
 
 template <int ST, int I>
 __device__ __forceinline__ void fast_load(const DevParams &P, const void *__restrict__ f, const float *__restrict__ bin,
-                                          const float *__restrict__ bout, const FastSrc &s, float &fr, float &fb) {
+                                          const float *__restrict__ bout, const FastSrc &s, float &fr, float &fb,
+                                          float &g) {
     constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
     const size_t nxz = size_t(P.NX) * P.NZ;
     if constexpr (cy > 0) {
@@ -86,6 +88,7 @@ __device__ __forceinline__ void fast_load(const DevParams &P, const void *__rest
             const int o = s.o - cz * P.NX - cx;
             fr = b[o];
             fb = b[nxz + o];
+            g = fr + fb;
             return;
         }
     }
@@ -95,6 +98,7 @@ __device__ __forceinline__ void fast_load(const DevParams &P, const void *__rest
             const int o = s.o - cz * P.NX - cx;
             fr = b[o];
             fb = b[nxz + o];
+            g = fr + fb;
             return;
         }
     }
@@ -105,7 +109,7 @@ __device__ __forceinline__ void fast_load(const DevParams &P, const void *__rest
         if constexpr (cz > 0) bb = bb || s.bzp;
         if constexpr (cz < 0) bb = bb || s.bzm;
         if (bb) {  // half-way bounce-back
-            Store<ST>::template load<opp(I)>(f, uint32_t(P.PX), s.p, fr, fb);
+            Store<ST>::template load<opp(I)>(f, uint32_t(P.PX), s.p, fr, fb, g);
             return;
         }
     }
@@ -117,7 +121,7 @@ __device__ __forceinline__ void fast_load(const DevParams &P, const void *__rest
     if constexpr (cy < 0) off += s.wym;
     if constexpr (cz > 0) off += s.wzp;
     if constexpr (cz < 0) off += s.wzm;
-    Store<ST>::template load<I>(f, uint32_t(P.PX), off, fr, fb);
+    Store<ST>::template load<I>(f, uint32_t(P.PX), off, fr, fb, g);
 }
 
 template <int ST>
@@ -127,9 +131,9 @@ __device__ __forceinline__ Moments fast_moments(const DevParams &P0, const DevBu
     MomentAcc acc;
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
-        float a, b;
-        fast_load<ST, I>(P, B.f_src, B.in_src, B.out_src, s, a, b);
-        acc.add<I>(a, b);
+        float a, b, g;
+        fast_load<ST, I>(P, B.f_src, B.in_src, B.out_src, s, a, b, g);
+        acc.add<I>(a, b, g);
     });
     return acc.finish(P.ghalf);
 }
@@ -187,11 +191,27 @@ __device__ __forceinline__ Moments raw_reduce(const RawGather<ST> &r, float ghal
     MomentAcc acc;
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
-        float a, b;
-        Store<ST>::template decode_raw<I>(r.w + I * Store<ST>::kWords, a, b);
-        acc.add<I>(a, b);
+        float a, b, g;
+        Store<ST>::template decode_raw<I>(r.w + I * Store<ST>::kWords, a, b, g);
+        acc.add<I>(a, b, g);
     });
     return acc.finish(ghalf);
+}
+
+// psi of a split gather from the colour sums alone, summed in MomentAcc's
+// order so that it is bitwise the psi of raw_reduce: all the halo lanes need
+// (a third of raw_reduce's arithmetic).
+template <int ST>
+__device__ __forceinline__ float raw_psi(const RawGather<ST> &r) {
+    float sr = 0.f, sb = 0.f;
+    unroll<0, Q>([&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        float a, b, g;
+        Store<ST>::template decode_raw<I>(r.w + I * Store<ST>::kWords, a, b, g);
+        sr += a;
+        sb += b;
+    });
+    return psi(sr, sb);
 }
 
 struct FusedGrid {
@@ -201,24 +221,17 @@ struct FusedGrid {
 #ifndef MCG_FUSED_WAVES_PER_EU
 #define MCG_FUSED_WAVES_PER_EU 3
 #endif
-// Wave roles.  Default: MCG_FUSED_TY = 9 core waves followed by 3 halo waves
-// (228 workgroups at 256x512x256).  MCG_FUSED_BALANCED=1: 8 core rows and 4
-// halo waves (the waves w with w % 3 == 2), which gives every SIMD 2 core
-// waves and 1 halo wave and 256 workgroups at 256x512x256; measured 6% (FP32)
-// and 24% (FP16) slower than the default on MI355X, so it is off.
-#ifndef MCG_FUSED_BALANCED
-#define MCG_FUSED_BALANCED 0
-#endif
-#if MCG_FUSED_BALANCED
-constexpr int kFusedTY = 8;
-constexpr int kFusedWaves = 12;
-#else
+// Waves: MCG_FUSED_TY core waves (one tile row each) followed by 3 halo waves.
+// The hardware deals the waves of a workgroup to the 4 SIMDs of a CU in turn
+// (wave w on SIMD w mod 4, read from the HW_ID register on MI355X), so with 8
+// core waves every SIMD runs 2 core waves (and 3 of them one halo wave); with 9
+// one SIMD would run 3 core waves, and the 16-bit step, whose arithmetic
+// nearly fills the SIMDs, ran 7% slower at 256x512x256.
 #ifndef MCG_FUSED_TY
-#define MCG_FUSED_TY 9
+#define MCG_FUSED_TY 8
 #endif
 constexpr int kFusedTY = MCG_FUSED_TY;
 constexpr int kFusedWaves = kFusedTY + 3;
-#endif
 
 // Tile of workgroup b when the hardware deals workgroups round-robin over the
 // 8 XCDs: XCD k gets a contiguous range of tiles, so tiles that share halo
@@ -265,15 +278,9 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     const int x0 = 1 + xt * 64, y0 = 1 + yb * TY;
     const int zb = 1 + zs * G.zlen;
     const int ze = min(P.NZ - 2, zb + G.zlen - 1);
-#if MCG_FUSED_BALANCED
-    const bool core = (wv % 3) != 2;
-    const int row = core ? wv - wv / 3 : 0;  // core row 0..7
-    const int hrole = core ? -1 : wv / 3;    // halo role 0..3
-#else
     const bool core = wv < TY;
     const int row = core ? wv : 0;
     const int hrole = core ? -1 : wv - TY;   // halo role 0..2
-#endif
     const int x = x0 + tx, y = y0 + row;
     const bool node = core && x <= P.NX - 2 && y <= P.NY - 2;
 
@@ -285,15 +292,9 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     } else if (hrole == 1) {
         lr = H - 1;
     } else if (hrole >= 2) {
-#if MCG_FUSED_BALANCED
-        active = tx < H;  // role 2: column x0-1, role 3: column x0+64
-        lr = tx;
-        lc = hrole == 2 ? 0 : W - 1;
-#else
         active = tx < 2 * H;
         lr = tx >> 1;
         lc = (tx & 1) ? W - 1 : 0;
-#endif
     }
     const int gx = x0 - 1 + lc, gy = y0 - 1 + lr;
 
@@ -367,7 +368,9 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         }
         if (lead && active) {
             float ph;
-            if (split) {
+            if (split && !core) {
+                ph = raw_psi<ST>(raw);  // a halo wave: only psi is used
+            } else if (split) {
                 nx2 = raw_reduce<ST>(raw, P.ghalf);
                 ph = psi(nx2.rr, nx2.rb);
             } else {
@@ -402,28 +405,62 @@ constexpr int fused_wg_per_cu() {
     return (4 * MCG_FUSED_WAVES_PER_EU) / kFusedWaves > 0 ? (4 * MCG_FUSED_WAVES_PER_EU) / kFusedWaves : 1;
 }
 
-// z segments: minimise rounds of resident workgroups x (segment length + 2 fill planes)
-inline FusedGrid fused_grid(const DevParams &P, int num_cu) {
-    FusedGrid best{};
-    double best_cost = 1e300;
-    const int nxt = P.PX / 64, nyb = (P.IY + kFusedTY - 1) / kFusedTY;
-    const int slots = std::max(1, num_cu * fused_wg_per_cu());
+// The launch of zs z segments: tiles of 64 x-columns by kFusedTY rows, each
+// marching through a segment of ceil(IZ/zs) planes.
+inline FusedGrid fused_grid_for(const DevParams &P, int zs) {
+    FusedGrid G;
+    G.nxt = P.PX / 64;
+    G.nyb = (P.IY + kFusedTY - 1) / kFusedTY;
+    G.zlen = (P.IZ + zs - 1) / zs;
+    G.nzs = (P.IZ + G.zlen - 1) / G.zlen;
+    G.nwg = G.nxt * G.nyb * G.nzs;
+    return G;
+}
+
+// Segment counts worth trying: every distinct segmentation with segments of
+// at least 8 planes (at most 16 of them, spread over the range).  Each
+// segment costs 2 fill planes of gathers, and the best count depends on how
+// the concurrently running tiles' rows fall on the memory channels: at
+// 256x512x256 the 16-bit step takes 2.14 ms with 17 segments, 2.27 ms with 12
+// or 16 and 3.06 ms with one (all workgroups loading the same planes at the
+// same time).  The solver times the candidates on its first steps and keeps
+// the fastest; the results do not depend on the segmentation.
+inline std::vector<FusedGrid> fused_candidates(const DevParams &P) {
+    std::vector<FusedGrid> all;
     const int maxseg = std::max(1, std::min(64, P.IZ / 8));
     for (int zs = 1; zs <= maxseg; ++zs) {
-        FusedGrid G;
-        G.nxt = nxt;
-        G.nyb = nyb;
-        G.zlen = (P.IZ + zs - 1) / zs;
-        G.nzs = (P.IZ + G.zlen - 1) / G.zlen;
-        G.nwg = G.nxt * G.nyb * G.nzs;
+        const FusedGrid G = fused_grid_for(P, zs);
+        if (all.empty() || all.back().nzs != G.nzs) all.push_back(G);
+    }
+    if (all.size() <= 16) return all;
+    std::vector<FusedGrid> pick;
+    for (int k = 0; k < 16; ++k) pick.push_back(all[size_t(k) * (all.size() - 1) / 15]);
+    return pick;
+}
+
+// The segmentation used before (or without) the timing: among the counts that
+// give at least kMinRounds rounds of resident workgroups (4 for FP32, 16 for
+// 16-bit storage), the fewest rounds x (segment length + 2 fill planes);
+// grids too small for that many rounds take the most segments.
+inline FusedGrid fused_grid(const DevParams &P, int num_cu) {
+    const int min_rounds = P.storage == MCG_STORE_FP32 ? 4 : 16;
+    const int slots = std::max(1, num_cu * fused_wg_per_cu());
+    FusedGrid best{}, most{};
+    double best_cost = 1e300;
+    for (const FusedGrid &G : fused_candidates(P)) {
+#ifdef MCG_FUSED_ZSEG
+        if (G.nzs != fused_grid_for(P, MCG_FUSED_ZSEG).nzs) continue;  // a fixed segment count (tuning builds)
+#endif
+        most = G;
         const int rounds = (G.nwg + slots - 1) / slots;
+        if (rounds < min_rounds) continue;
         const double cost = double(rounds) * double(G.zlen + 2);
         if (cost < best_cost) {
             best_cost = cost;
             best = G;
         }
     }
-    return best;
+    return best_cost < 1e300 ? best : most;
 }
 
 inline size_t fused_stats_lds(const DevParams &P) { return size_t(kFusedTY) * P.NR * 8 * sizeof(double); }
@@ -434,8 +471,7 @@ inline bool fused_has_stats(const DevParams &P, size_t lds_per_block) {
 }
 
 template <int ST>
-inline int launch_fused_st(const DevParams &P, const DevBufs &B, hipStream_t stream, int num_cu) {
-    const FusedGrid G = fused_grid(P, num_cu);
+inline int launch_fused_st(const DevParams &P, const DevBufs &B, hipStream_t stream, const FusedGrid &G) {
     const dim3 block(64, kFusedWaves, 1);
     if (P.wall || P.grav != 0.0f) {
         if (B.do_stats) {
@@ -465,11 +501,11 @@ inline int fused_allow_lds(size_t bytes) {
                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
 }
 
-inline int launch_fused(const DevParams &P, const DevBufs &B, hipStream_t stream, int num_cu) {
+inline int launch_fused(const DevParams &P, const DevBufs &B, hipStream_t stream, const FusedGrid &G) {
     switch (P.storage) {
-        case MCG_STORE_FP16: return launch_fused_st<MCG_STORE_FP16>(P, B, stream, num_cu);
-        case MCG_STORE_BF16: return launch_fused_st<MCG_STORE_BF16>(P, B, stream, num_cu);
-        default: return launch_fused_st<MCG_STORE_FP32>(P, B, stream, num_cu);
+        case MCG_STORE_FP16: return launch_fused_st<MCG_STORE_FP16>(P, B, stream, G);
+        case MCG_STORE_BF16: return launch_fused_st<MCG_STORE_BF16>(P, B, stream, G);
+        default: return launch_fused_st<MCG_STORE_FP32>(P, B, stream, G);
     }
 }
 

@@ -81,11 +81,12 @@ struct MomentAcc {
     float sr = 0.f, sb = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
     float sxx = 0.f, sxy = 0.f, syy = 0.f, syz = 0.f, szz = 0.f, sxz = 0.f;
 
+    // g: the two-colour population f_r + f_b of direction I (16-bit storage
+    // hands over the stored total directly)
     template <int I>
-    __device__ __forceinline__ void add(float fr, float fb) {
+    __device__ __forceinline__ void add(float fr, float fb, float g) {
         sr += fr;
         sb += fb;
-        const float g = fr + fb;
         if constexpr (kCX[I] > 0) jx += g; else if constexpr (kCX[I] < 0) jx -= g;
         if constexpr (kCY[I] > 0) jy += g; else if constexpr (kCY[I] < 0) jy -= g;
         if constexpr (kCZ[I] > 0) jz += g; else if constexpr (kCZ[I] < 0) jz -= g;
@@ -173,10 +174,14 @@ struct NodeColl {
 
 __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// BULK: the node carries no Omega2 or recolouring (s2 = sB = rc = 0, F = 0),
+// so those terms, each an exact zero contribution, are left out; the values
+// are those of the general form (a zero may differ in sign).
 // rest direction: r = aR g0, b = aB g0
+template <bool BULK = false>
 __device__ __forceinline__ void collide_rest(const NodeColl &n, float &r, float &b) {
     float E = fma_(n.q2, n.tr, n.e0);
-    E = fma_(n.sB, 1.25f, E);
+    if constexpr (!BULK) E = fma_(n.sB, 1.25f, E);
     const float g = w(0) * E;
     r = n.aR * g;
     b = n.aB * g;
@@ -184,7 +189,7 @@ __device__ __forceinline__ void collide_rest(const NodeColl &n, float &r, float 
 
 // Even and odd parts of the pair (I, I+1) divided by w_I: E, O and the
 // recolouring term D (directions 2k-1 and 2k are opposite).
-template <int I>
+template <int I, bool BULK = false>
 __device__ __forceinline__ void pair_terms(const NodeColl &n, float &E, float &O, float &D) {
     static_assert(I % 2 == 1 && kCX[I] == -kCX[I + 1] && kCY[I] == -kCY[I + 1] && kCZ[I] == -kCZ[I + 1], "pair");
     const float cu = cdot<I>(n.ux, n.uy, n.uz);
@@ -204,10 +209,13 @@ __device__ __forceinline__ void pair_terms(const NodeColl &n, float &E, float &O
     if constexpr (Hyyz(I) != 0.0f) a3 = fma_(n.tyyz, Hyyz(I), a3);
     if constexpr (Hyzz(I) != 0.0f) a3 = fma_(n.tyzz, Hyzz(I), a3);
     if constexpr (Hxyz(I) != 0.0f) a3 = fma_(n.txyz, Hxyz(I), a3);
-    const float d = cdot<I>(n.Fx, n.Fy, n.Fz);
     E = fma_(n.e1, cu2, n.e0);
     E = fma_(n.q2, a2, E);
-    E = fma_(n.s2, d * d, E) - n.sB;
+    float d = 0.0f;
+    if constexpr (!BULK) {
+        d = cdot<I>(n.Fx, n.Fy, n.Fz);
+        E = fma_(n.s2, d * d, E) - n.sB;
+    }
     O = cu * fma_(n.o3, cu2, n.o1);
     O = fma_(n.q3, a3, O);
     // body force F = (0, F_y, 0): even part (1 - omega/2)(9 (c.u)(c.F) - 3 u.F) (its
@@ -221,39 +229,57 @@ __device__ __forceinline__ void pair_terms(const NodeColl &n, float &E, float &O
             O = O - n.g3;
         }
     }
-    D = n.rc * d;
-    if constexpr (csq(I) > 1) D = D * invc(I);
+    if constexpr (BULK) {
+        D = 0.0f;
+    } else {
+        D = n.rc * d;
+        if constexpr (csq(I) > 1) D = D * invc(I);
+    }
 }
 
 // direction I and its opposite I+1
-template <int I>
+template <int I, bool BULK = false>
 __device__ __forceinline__ void collide_pair(const NodeColl &n, float &rp, float &bp, float &rm, float &bm) {
     float E, O, D;
-    pair_terms<I>(n, E, O, D);
+    pair_terms<I, BULK>(n, E, O, D);
     const float gp = w(I) * (E + O);
     const float gm = w(I) * (E - O);
-    const float Dw = w(I) * D;
-    rp = fma_(n.aR, gp, Dw);
-    bp = fma_(n.aB, gp, -Dw);
-    rm = fma_(n.aR, gm, -Dw);
-    bm = fma_(n.aB, gm, Dw);
+    if constexpr (BULK) {
+        rp = n.aR * gp;
+        bp = n.aB * gp;
+        rm = n.aR * gm;
+        bm = n.aB * gm;
+    } else {
+        const float Dw = w(I) * D;
+        rp = fma_(n.aR, gp, Dw);
+        bp = fma_(n.aB, gp, -Dw);
+        rm = fma_(n.aR, gm, -Dw);
+        bm = fma_(n.aB, gm, Dw);
+    }
 }
 
 // The same pair for 16-bit storage, which keeps populations divided by w_I:
 // the offsets (E +- O) - 1 and the two colours (E +- O) aR +- D, (E +- O) aB -+ D.
-template <int I>
+template <int I, bool BULK = false>
 __device__ __forceinline__ void collide_pair_w(const NodeColl &n, float &ap, float &rp, float &bp, float &am, float &rm,
                                                float &bm) {
     float E, O, D;
-    pair_terms<I>(n, E, O, D);
+    pair_terms<I, BULK>(n, E, O, D);
     const float gp = E + O, gm = E - O;
     const float em1 = E - 1.0f;
     ap = em1 + O;
     am = em1 - O;
-    rp = fma_(n.aR, gp, D);
-    bp = fma_(n.aB, gp, -D);
-    rm = fma_(n.aR, gm, -D);
-    bm = fma_(n.aB, gm, D);
+    if constexpr (BULK) {
+        rp = n.aR * gp;
+        bp = n.aB * gp;
+        rm = n.aR * gm;
+        bm = n.aB * gm;
+    } else {
+        rp = fma_(n.aR, gp, D);
+        bp = fma_(n.aB, gp, -D);
+        rm = fma_(n.aR, gm, -D);
+        bm = fma_(n.aB, gm, D);
+    }
 }
 
 // Regularised boundary population (inlet jet/boundary_conditions.cuh:76-79,

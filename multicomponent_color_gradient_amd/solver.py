@@ -65,6 +65,17 @@ class Solver:
         self.kernel = kernel
 
     # ---- case API (jet/jetIn.cuh, bubble/bubbleIn.cuh)
+    def set_launch_tuning(self, enable: bool) -> None:
+        """Time the candidate z segmentations of the step kernel on the first plain
+        steps and keep the fastest (default), or use the fixed rule (False)."""
+        check(self.lib.mcg_set_launch_tuning(self._h, 1 if enable else 0))
+
+    def launch_info(self) -> dict:
+        """The step kernel's launch shape: z segments, workgroups per step, tuned or not."""
+        seg, wg, tuned = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(self.lib.mcg_launch_info(self._h, ctypes.byref(seg), ctypes.byref(wg), ctypes.byref(tuned)))
+        return {"segments": seg.value, "workgroups": wg.value, "tuned": bool(tuned.value)}
+
     def init_jet(self) -> None:
         check(self.lib.mcg_init_jet(self._h))
 

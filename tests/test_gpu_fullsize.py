@@ -93,3 +93,26 @@ def test_checkpoint_roundtrip_at_full_size():
     del buf
     s.step(5)
     assert np.array_equal(s.fields()["uy"], a)
+
+
+def test_launch_tuning_keeps_results_bitwise():
+    """The step kernel's z segmentation is timed on the first plain steps and
+    the fastest kept; that choice changes no result: after 60 steps (16-bit
+    storage, statistics every 10 steps) the fields and the radial sums equal
+    those of a run with the fixed rule bit for bit."""
+    c = jet256()
+    out = []
+    for tune in (True, False):
+        print(f"full size: fp16 jet 256x512x256, 60 steps, launch tuning {tune}", flush=True)
+        s = m.Solver(c, storage="fp16")
+        s.set_launch_tuning(tune)
+        s.init_jet()
+        s.step(60)
+        f = s.fields()
+        out.append(({k: f[k] for k in ("rhor", "rhob", "ux", "uy", "uz")}, s.radial()["count"], s.launch_info()))
+        s.close()
+    (fa, ca, ia), (fb, cb, ib) = out
+    print(f"launch shapes: tuned {ia}, fixed {ib}", flush=True)
+    assert ia["tuned"] and not ib["tuned"]
+    assert all(np.array_equal(fa[k], fb[k]) for k in fa)
+    assert np.array_equal(ca, cb)
