@@ -143,12 +143,12 @@ int mcg_set_kernel(mcg_solver *s, int kernel);
 
 /* Launch shape of the fused step (an extension; the reference launches fixed
  * block shapes, utilities/cudaConfig.cuh:22-31).  The step kernel cuts z into
- * segments; by default the solver's first plain steps (no statistics) try the
- * candidate segment counts in turn, each timed with HIP events and waited for,
- * and keep the fastest of two passes.  The results do not depend on the
- * segmentation.  enable = 0 uses a fixed rule instead and never waits; 1
- * (re)starts the timing. */
-int mcg_set_launch_tuning(mcg_solver *s, int enable);
+ * segments.  segments = 0 (the default): the solver's first plain steps (no
+ * statistics) try the candidate segment counts in turn, each timed with HIP
+ * events and waited for, and keep the fastest of two passes.  segments > 0:
+ * the candidate nearest to that many segments, fixed, with no timing.  The
+ * results do not depend on the segmentation. */
+int mcg_set_launch_segments(mcg_solver *s, int segments);
 
 /* The current launch shape: z segments, workgroups per step, and whether the
  * timing has chosen it.  Any pointer may be NULL. */

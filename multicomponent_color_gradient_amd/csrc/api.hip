@@ -660,13 +660,21 @@ int mcg_set_kernel(mcg_solver *s, int kernel) {
     return MCG_OK;
 }
 
-int mcg_set_launch_tuning(mcg_solver *s, int enable) {
+int mcg_set_launch_segments(mcg_solver *s, int segments) {
     if (int rc = check_solver(s)) return rc;
-    s->tune_on = enable != 0;
+    if (segments < 0) return fail(MCG_E_INVALID, "segments must be >= 0");
     s->tuned = false;
     s->tune_next = 0;
     s->tune_ms.clear();
+    s->tune_on = segments == 0;
     s->grid = fused_grid(s->P, s->num_cu);
+    if (segments > 0) {
+        // the candidate with the segment count nearest to the one asked for
+        const FusedGrid *best = &s->tune_cand.front();
+        for (const FusedGrid &G : s->tune_cand)
+            if (std::abs(G.nzs - segments) < std::abs(best->nzs - segments)) best = &G;
+        s->grid = *best;
+    }
     return MCG_OK;
 }
 

@@ -65,10 +65,10 @@ class Solver:
         self.kernel = kernel
 
     # ---- case API (jet/jetIn.cuh, bubble/bubbleIn.cuh)
-    def set_launch_tuning(self, enable: bool) -> None:
-        """Time the candidate z segmentations of the step kernel on the first plain
-        steps and keep the fastest (default), or use the fixed rule (False)."""
-        check(self.lib.mcg_set_launch_tuning(self._h, 1 if enable else 0))
+    def set_launch_segments(self, segments: int = 0) -> None:
+        """0: time the candidate z segmentations of the step kernel on the first
+        plain steps and keep the fastest (default); n > 0: about n segments, fixed."""
+        check(self.lib.mcg_set_launch_segments(self._h, int(segments)))
 
     def launch_info(self) -> dict:
         """The step kernel's launch shape: z segments, workgroups per step, tuned or not."""

@@ -173,6 +173,7 @@ struct OracleBase {
     virtual int64_t get_step() = 0;
     virtual void set_inlet_offset(const double *du) = 0;
     virtual double uy_dot(const double *w) = 0;
+    virtual void branch_masks(uint8_t *out) = 0;
 };
 
 template <class T>
@@ -390,6 +391,32 @@ struct Oracle final : OracleBase {
         Fx = L.inv_cs2 * sx;
         Fy = L.inv_cs2 * sy;
         Fz = L.inv_cs2 * sz;
+    }
+
+    // The branches the collision of the last step took, per interior node (x
+    // fastest): bit 0 = interface (I > 1e-4, lbm.cuh:214-216), bit 1 = |F| >
+    // 1e-4 at an interface node (the cut of forces.cuh:61-64 and
+    // collisionOperators.cuh:36-39 not taken).  The same tests as collistream,
+    // on the moments it used.
+    void branch_masks(uint8_t *out) override {
+        const size_t ix = size_t(NX - 2), iy = size_t(NY - 2);
+#pragma omp parallel for schedule(static)
+        for (int z = 1; z <= NZ - 2; ++z)
+            for (int y = 1; y <= NY - 2; ++y)
+                for (int x = 1; x <= NX - 2; ++x) {
+                    const size_t id = idx(x, y, z);
+                    const T rr = rhor[id], rb = rhob[id];
+                    const T invrT = T(1) / (rr + rb);
+                    const T I = T(4) * rr * rb * invrT * invrT;
+                    uint8_t m = 0;
+                    if (!(I <= T(1e-4))) {
+                        T Fx, Fy, Fz;
+                        force(x, y, z, Fx, Fy, Fz);
+                        const T absF = std::sqrt(Fx * Fx + Fy * Fy + Fz * Fz);
+                        m = absF <= T(1.0e-4) ? 1 : 3;
+                    }
+                    out[size_t(x - 1) + ix * (size_t(y - 1) + iy * size_t(z - 1))] = m;
+                }
     }
 
     // Collision + perturbation + recolouring + push streaming, lbm.cuh:184-293
@@ -777,6 +804,7 @@ int mcgo_writeset(void *h, uint8_t *cr, uint8_t *cb) { return static_cast<Oracle
 void mcgo_gradient(void *h, int x, int y, int z, double *F) { static_cast<OracleBase *>(h)->gradient(x, y, z, F); }
 void mcgo_set_inlet_offset(void *h, const double *du) { static_cast<OracleBase *>(h)->set_inlet_offset(du); }
 double mcgo_uy_dot(void *h, const double *w) { return static_cast<OracleBase *>(h)->uy_dot(w); }
+void mcgo_branch_masks(void *h, uint8_t *out) { static_cast<OracleBase *>(h)->branch_masks(out); }
 
 // ---- single-node operators, exposed for the invariant tests (FP64) ----
 static const Lattice<double> &lat() {

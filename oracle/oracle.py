@@ -41,6 +41,7 @@ _SIGS = [
     ("mcgo_gradient", None, [c_void_p, c_int, c_int, c_int, _D]),
     ("mcgo_set_inlet_offset", None, [c_void_p, _D]),
     ("mcgo_uy_dot", c_double, [c_void_p, _D]),
+    ("mcgo_branch_masks", None, [c_void_p, POINTER(c_uint8)]),
     ("mcgo_lattice", None, [POINTER(c_int), POINTER(c_int), POINTER(c_int), _D, _D, _D]),
     ("mcgo_feq", c_double, [c_int, c_double, c_double, c_double, c_double]),
     ("mcgo_fneqr", c_double, [c_int, _D, c_double, c_double, c_double]),
@@ -217,6 +218,13 @@ class Oracle:
         moments pass of the last step: a u_y functional without copying the fields."""
         w = np.ascontiguousarray(w, np.float64).reshape(self.shape)
         return lib().mcgo_uy_dot(self._h, _p(w))
+
+    def branch_masks(self):
+        """Per interior node (shape (NZ-2, NY-2, NX-2)) the branches the last step's
+        collision took: bit 0 interface (I > 1e-4), bit 1 |F| > 1e-4 there."""
+        out = np.zeros((self.case.NZ - 2, self.case.NY - 2, self.case.NX - 2), np.uint8)
+        lib().mcgo_branch_masks(self._h, out.ctypes.data_as(POINTER(c_uint8)))
+        return out
 
     def gradient(self, x, y, z):
         F = np.zeros(3)

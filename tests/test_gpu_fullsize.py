@@ -99,20 +99,20 @@ def test_launch_tuning_keeps_results_bitwise():
     """The step kernel's z segmentation is timed on the first plain steps and
     the fastest kept; that choice changes no result: after 60 steps (16-bit
     storage, statistics every 10 steps) the fields and the radial sums equal
-    those of a run with the fixed rule bit for bit."""
+    those of runs with 1 and 7 fixed segments bit for bit."""
     c = jet256()
     out = []
-    for tune in (True, False):
-        print(f"full size: fp16 jet 256x512x256, 60 steps, launch tuning {tune}", flush=True)
+    for seg in (0, 1, 7):
+        print(f"full size: fp16 jet 256x512x256, 60 steps, segments {seg or 'timed'}", flush=True)
         s = m.Solver(c, storage="fp16")
-        s.set_launch_tuning(tune)
+        s.set_launch_segments(seg)
         s.init_jet()
         s.step(60)
         f = s.fields()
         out.append(({k: f[k] for k in ("rhor", "rhob", "ux", "uy", "uz")}, s.radial()["count"], s.launch_info()))
         s.close()
-    (fa, ca, ia), (fb, cb, ib) = out
-    print(f"launch shapes: tuned {ia}, fixed {ib}", flush=True)
-    assert ia["tuned"] and not ib["tuned"]
-    assert all(np.array_equal(fa[k], fb[k]) for k in fa)
-    assert np.array_equal(ca, cb)
+    print(f"launch shapes: {[o[2] for o in out]}", flush=True)
+    assert out[0][2]["tuned"] and not out[1][2]["tuned"] and out[1][2]["segments"] == 1
+    for fa, ca, _ in out[1:]:
+        assert all(np.array_equal(out[0][0][k], fa[k]) for k in fa)
+        assert np.array_equal(out[0][1], ca)
