@@ -14,7 +14,10 @@ MLUPS follows utilities/countmlups.cuh:42-48 (steps * NX*NY*NZ / t, ghost
 layers counted); the interior-only figure is reported too.  Defaults: 200
 warm-up steps, then 1000 timed steps in 5 windows of 200 (HIP events on the
 solver's stream around each); `value` is the rate over all 1000, and the
-median window's rate is reported beside it.  The roofline
+median window's rate is reported beside it.  Each solver first times the
+candidate z segmentations of its step kernel and keeps the fastest (untimed
+steps after the warm-up until it has chosen; `config.launch` gives the shape
+and how many steps the choice took); `--segments n` fixes the shape instead.  The roofline
 uses 432 algorithmic bytes per interior lattice update (27 populations x 2
 colours read + written, FP32; 216 B with 16-bit storage) against 8 TB/s.
 
@@ -90,6 +93,17 @@ def copy_probe(nbytes: int, reps: int = 3) -> None:
         hip.hipFree(b)
 
 
+def settle_launch(s, limit: int = 200) -> int:
+    """Untimed steps until the solver's launch-shape timing (about 2 steps per
+    candidate segmentation, on steps without statistics) has chosen; returns them."""
+    n = 0
+    while not s.launch_info()["tuned"] and n < limit:
+        s.step(4)
+        n += 4
+    s.synchronize()
+    return n
+
+
 PROBE_KINDS = {"read": 0, "write": 1, "copy": 2}
 
 
@@ -137,6 +151,7 @@ def adjoint_ratio(steps: int, every: int) -> dict:
     s = m.Solver(c)
     s.init_bubble_tanh()
     s.step(10)
+    settle_launch(s)
     fwd_ms = s.time_steps(steps)
     s.adjoint_record(steps, every)
     fb = s.populations()[1]
@@ -164,6 +179,7 @@ def adjoint_jet_ratio(steps: int, every: int) -> dict:
     s = m.Solver(c)
     s.init_jet()
     s.step(10)
+    settle_launch(s)
     fwd_ms = s.time_steps(steps)
     w = np.zeros(s.shape, np.float32)
     w[int(c.jet_z0), 1:-1, int(c.jet_x0)] = 1.0
@@ -200,6 +216,7 @@ def interface_rich(steps: int, warmup: int, kernel: str) -> dict:
         print(f"bench: interface-rich 256x512x256 storage={storage} warmup={warmup} steps={steps}", file=sys.stderr,
               flush=True)
         s.step(warmup)
+        settle_launch(s)
         ms = s.time_steps(steps)
         s.check_health()
         if storage == "fp32":
@@ -285,6 +302,9 @@ def main():
     ap.add_argument("--no-interface-rich", action="store_true", help="skip the interface-rich (256 spheres) run")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=200)
+    ap.add_argument("--segments", type=int, default=0,
+                    help="z segments of the step kernel (0: the solver times the candidates on its first steps, "
+                         "the default; n: about n, fixed, e.g. to profile the shape a default run chose)")
     ap.add_argument("--ceiling", action="store_true",
                     help="only measure the HBM bandwidth of plain read, write and copy streams (every probe "
                          "variant) and print it as one JSON line")
@@ -308,10 +328,13 @@ def main():
 
     def measure(storage: str) -> dict:
         s = m.Solver(case, storage=storage, kernel=args.kernel)
+        s.set_launch_segments(args.segments)
         getattr(s, "init_" + init)()
         print(f"bench: {args.workload} {case.NX}x{case.NY}x{case.NZ} storage={storage} kernel={args.kernel} "
               f"warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
         s.step(args.warmup)
+        # the launch-shape timing finishes before the timed steps, whatever --warmup says
+        shape_steps = settle_launch(s) if args.segments == 0 else 0
         s.synchronize()
         # the K timed steps run as WINDOWS consecutive windows, each bracketed by
         # HIP events on the solver's stream; the total is the sum of the windows
@@ -333,7 +356,7 @@ def main():
                 "avg_ms": avg_ms, "wall": wall, "bpl": bpl, "achieved": achieved,
                 "traffic": load_traffic(args.workload, storage),
                 "windows": [round(x, 5) for x in per_window], "median_ms": float(sorted(per_window)[len(per_window) // 2]),
-                "launch": launch}
+                "launch": dict(launch, shape_timing_steps=shape_steps)}
 
     r = measure(args.storage)
     mlups, mlups_int, avg_ms, wall = r["mlups"], r["mlups_int"], r["avg_ms"], r["wall"]

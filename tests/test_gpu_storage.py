@@ -227,3 +227,33 @@ def test_16bit_checkpoint_restart_is_bitwise(mcg, storage):
     c = t.populations()
     t.close()
     assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
+def test_census_counts_negative_populations(mcg):
+    """census.negative_counts against numpy's own count on populations set with
+    a known pattern of negatives (FP32 storage keeps every value)."""
+    from multicomponent_color_gradient_amd.census import negative_counts
+
+    case = mcg.jet_case(20, 16, 18, jet_radius=3.0)
+    s = mcg.Solver(case)
+    s.init_jet()
+    w = weights()
+    shape = (27,) + s.shape
+    rng = np.random.default_rng(8)
+    fr = (w * (0.5 + rng.random(shape))).astype(np.float32)
+    fb = (w * rng.random(shape)).astype(np.float32)
+    neg_r = rng.random(shape) < 0.01
+    neg_b = rng.random(shape) < 0.02
+    fr[neg_r] *= -1.0
+    fb[neg_b] *= -3.0
+    s.set_populations(fr, fb)
+    c = negative_counts(s)
+    s.close()
+    sl = (slice(None),) + INNER
+    r, b = fr[sl], fb[sl]
+    minority = np.where(np.abs(r) < np.abs(b), r, b)
+    assert c["nodes"] == int(np.prod(r.shape[1:]))
+    assert c["fr_negative"] == np.count_nonzero((r < 0).reshape(27, -1), axis=1).tolist()
+    assert c["fb_negative"] == np.count_nonzero((b < 0).reshape(27, -1), axis=1).tolist()
+    assert c["g_negative"] == np.count_nonzero(((r + b) < 0).reshape(27, -1), axis=1).tolist()
+    assert c["minority_negative_total"] == int(np.count_nonzero(minority < 0)) > 0
