@@ -19,7 +19,9 @@ candidate z segmentations of its step kernel and keeps the fastest (untimed
 steps after the warm-up until it has chosen; `config.launch` gives the shape
 and how many steps the choice took); `--segments n` fixes the shape instead.  The roofline
 uses 432 algorithmic bytes per interior lattice update (27 populations x 2
-colours read + written, FP32; 216 B with 16-bit storage) against 8 TB/s.
+colours read + written, FP32; 216 B with 16-bit storage) against 8 TB/s;
+`hbm_ceiling` gives what plain read, write and copy kernels reach on the same
+device, and the step's measured HBM traffic per second against the copy.
 
 The CPU baseline runs the FP64 OpenMP oracle (oracle/, a restatement of the
 reference step) on the same grid for a bounded number of steps.
@@ -305,6 +307,8 @@ def main():
     ap.add_argument("--segments", type=int, default=0,
                     help="z segments of the step kernel (0: the solver times the candidates on its first steps, "
                          "the default; n: about n, fixed, e.g. to profile the shape a default run chose)")
+    ap.add_argument("--no-ceiling", action="store_true",
+                    help="skip the short read/write/copy bandwidth probe reported beside the step")
     ap.add_argument("--ceiling", action="store_true",
                     help="only measure the HBM bandwidth of plain read, write and copy streams (every probe "
                          "variant) and print it as one JSON line")
@@ -409,6 +413,14 @@ def main():
     if not args.no_adjoint and args.storage == "fp32":
         out["adjoint"] = adjoint_ratio(args.adjoint_steps, args.adjoint_every)
         out["adjoint_jet"] = adjoint_jet_ratio(args.adjoint_steps, args.adjoint_every)
+    if not args.no_ceiling:
+        # what plain streams reach on this device, beside what the step moves
+        ceil = hbm_ceiling(full=False, reps=5)["best_gbs"]
+        moved = traffic / (avg_ms / 1e3) / 1e9 if traffic else None
+        out["hbm_ceiling"] = {"unit": "GB/s", "read": ceil["read"], "write": ceil["write"], "copy": ceil["copy"],
+                              "step_traffic_gbs": round(moved, 1) if moved else None,
+                              "step_traffic_over_copy": round(moved / ceil["copy"], 3) if moved else None,
+                              "probe": "16 B per lane, 8 workgroups per CU, 4 GiB buffers, best of 5"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, init, args.cpu_seconds)
     else:

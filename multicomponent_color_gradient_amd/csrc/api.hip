@@ -101,6 +101,7 @@ struct mcg_solver {
     float *grr = nullptr, *grb = nullptr;
     float *mom = nullptr;
     double *stats = nullptr;
+    unsigned long long *stats_fx = nullptr;  // one step's sums in fixed point, zero between steps
     unsigned long long *count = nullptr;
     double *red_partials = nullptr;
     double *red_out = nullptr;
@@ -228,7 +229,7 @@ struct mcg_solver {
         B.mom = mom;
         B.mom_out = rec_mom;
         B.inlet_du = inlet_du;
-        B.stats = stats;
+        B.stats_fx = stats_fx;
         B.count = count;
         B.do_stats = do_stats ? 1 : 0;
         B.step = (unsigned long long)(step);
@@ -366,6 +367,7 @@ int do_init(mcg_solver *s, int kind) {
     HIP_TRY(hipMemcpyAsync(s->bin[1], s->bin[0], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->bout[1], s->bout[0], s->bnd_bytes(), hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipMemsetAsync(s->stats, 0, sizeof(double) * 7 * s->bins(), s->stream));
+    HIP_TRY(hipMemsetAsync(s->stats_fx, 0, sizeof(unsigned long long) * 7 * s->bins(), s->stream));
     HIP_TRY(hipMemsetAsync(s->count, 0, sizeof(unsigned long long) * s->bins(), s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->step = 0;
@@ -440,8 +442,8 @@ int one_step(mcg_solver *s) {
         if (rc) return rc;
         if (do_stats) {
             const dim3 g(unsigned(s->P.IY), unsigned((s->P.IZ + kRadialZChunk - 1) / kRadialZChunk));
-            hipLaunchKernelGGL(k_radial, g, dim3(256), size_t(8) * s->P.NR * sizeof(double), s->stream, s->P, s->mom,
-                               s->stats, s->count);
+            hipLaunchKernelGGL(k_radial, g, dim3(256), size_t(8) * s->P.NR * sizeof(unsigned long long), s->stream, s->P,
+                               s->mom, s->stats_fx, s->count);
             LAUNCH_CHECK("k_radial");
         }
         MCG_DISPATCH_ST(s->storage, k_collide, s->grid_interior(), dim3(64, 4, 1), 0, s->stream, s->P, B);
@@ -455,8 +457,8 @@ int one_step(mcg_solver *s) {
             int rc = launch_moments(s, s->cur);
             if (rc) return rc;
             const dim3 g(unsigned(s->P.IY), unsigned((s->P.IZ + kRadialZChunk - 1) / kRadialZChunk));
-            hipLaunchKernelGGL(k_radial, g, dim3(256), size_t(8) * s->P.NR * sizeof(double), s->stream, s->P, s->mom,
-                               s->stats, s->count);
+            hipLaunchKernelGGL(k_radial, g, dim3(256), size_t(8) * s->P.NR * sizeof(unsigned long long), s->stream, s->P,
+                               s->mom, s->stats_fx, s->count);
             LAUNCH_CHECK("k_radial");
             Bf.do_stats = 0;
             have_moments = true;
@@ -475,7 +477,13 @@ int one_step(mcg_solver *s) {
                            s->obs_acc);
         LAUNCH_CHECK("k_adj_sum");
     }
-    if (do_stats) ++s->samples;  // never during adjoint recomputation (stats_frozen)
+    if (do_stats) {
+        // this step's fixed-point sums into the FP64 sums, in a fixed order
+        hipLaunchKernelGGL(k_stats_fold, dim3(unsigned((s->bins() + 255) / 256)), dim3(256), 0, s->stream, s->bins(),
+                           s->stats_fx, s->stats);
+        LAUNCH_CHECK("k_stats_fold");
+        ++s->samples;  // never during adjoint recomputation (stats_frozen)
+    }
     if (s->inlet_reset_pending) {
         hipLaunchKernelGGL(k_inlet_ghost_reset, dim3(unsigned((s->c.NX - 2 + 255) / 256), unsigned(s->c.NZ - 2)), dim3(256), 0,
                            s->stream, s->P, s->grr, s->grb);
@@ -540,6 +548,8 @@ size_t mcg_case_size(void) { return sizeof(mcg_case); }
 int mcg_case_validate(const mcg_case *c) {
     if (!c) return fail(MCG_E_INVALID, "null case");
     if (c->NX < 4 || c->NY < 3 || c->NZ < 4) return fail(MCG_E_INVALID, "grid too small (NX,NZ >= 4, NY >= 3)");
+    // (also keeps a radial bin under about 8 NX = 16384 samples per step, far inside
+    // the 2^20 the fixed-point statistics sums hold)
     if (c->NX > 2048) return fail(MCG_E_INVALID, "NX > 2048 (radial bins NX/2 must fit the LDS histogram)");
     const size_t px = size_t((c->NX - 2 + 63) / 64 * 64);
     if ((px * 54 + kRowPad) * size_t(c->NY - 2) * size_t(c->NZ - 2) >= (size_t(1) << 32))
@@ -622,6 +632,7 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
     ok = ok && alloc((void **)&s->grb, s->N() * sizeof(float));
     ok = ok && alloc((void **)&s->mom, size_t(11) * s->P.plane * sizeof(float));
     ok = ok && alloc((void **)&s->stats, sizeof(double) * 7 * s->bins());
+    ok = ok && alloc((void **)&s->stats_fx, sizeof(unsigned long long) * 7 * s->bins());
     ok = ok && alloc((void **)&s->count, sizeof(unsigned long long) * s->bins());
     s->red_blocks = 1024;
     ok = ok && alloc((void **)&s->red_partials, sizeof(double) * kRedQ * s->red_blocks);
@@ -629,6 +640,13 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
     if (!ok) {
         mcg_destroy(s);
         return fail(MCG_E_NOMEM, "device allocation failed");
+    }
+    // the fixed-point statistics sums are zero between steps from here on, also
+    // for a solver that starts from a checkpoint rather than an initial state
+    if (hipMemsetAsync(s->stats_fx, 0, sizeof(unsigned long long) * 7 * s->bins(), s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess) {
+        mcg_destroy(s);
+        return fail(MCG_E_HIP, "hipMemset of the statistics sums failed");
     }
     *out = s;
     return MCG_OK;
@@ -640,8 +658,8 @@ int mcg_destroy(mcg_solver *s) {
         for (void *p : {s->f[b], (void *)s->bin[b], (void *)s->bout[b], (void *)s->gB[b]})
             if (p) (void)hipFree(p);
     }
-    for (void *p : {(void *)s->grr, (void *)s->grb, (void *)s->mom, (void *)s->stats, (void *)s->count,
-                    (void *)s->red_partials, (void *)s->red_out})
+    for (void *p : {(void *)s->grr, (void *)s->grb, (void *)s->mom, (void *)s->stats, (void *)s->stats_fx,
+                    (void *)s->count, (void *)s->red_partials, (void *)s->red_out})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : s->tune_ev)
         if (e) (void)hipEventDestroy(e);
@@ -761,6 +779,7 @@ int mcg_stats_info(const mcg_solver *s, int32_t *started, int64_t *start_step, i
 int mcg_stats_reset(mcg_solver *s) {
     if (int rc = check_ready(s)) return rc;
     HIP_TRY(hipMemsetAsync(s->stats, 0, sizeof(double) * 7 * s->bins(), s->stream));
+    HIP_TRY(hipMemsetAsync(s->stats_fx, 0, sizeof(unsigned long long) * 7 * s->bins(), s->stream));
     HIP_TRY(hipMemsetAsync(s->count, 0, sizeof(unsigned long long) * s->bins(), s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->started = false;

@@ -76,11 +76,36 @@ struct DevBufs {
     float *mom;            // 11 planes
     float *mom_out;        // if set: the moments of every node of this step's source state (11 planes)
     const float *inlet_du;  // if set: offset of the inlet velocity per column (3 x NX*NZ, component-major)
-    double *stats;         // 7 planes of NY*NR (uy, uy2, ur, ur2, uruy, phi, phi2)
+    unsigned long long *stats_fx;  // 7 planes of NY*NR (uy, uy2, ur, ur2, uruy, phi, phi2): this step's sums, fixed point
     unsigned long long *count;  // NY*NR
     int do_stats;
     unsigned long long step;    // index of the step being computed (0 for the first)
 };
+
+// Radial-statistics samples are summed as 64-bit two's-complement integers in
+// units of 2^-36.  Integer adds give the same sum in any order, so the sums of a
+// step do not depend on the order in which the atomics land; k_stats_fold then
+// adds each step's sums to the FP64 running sums in a fixed order.  A summand is
+// clamped to +-128 (lattice velocities and psi stay below 1, NaN goes to the
+// lower bound), so a bin can take 2^20 samples per step without overflow.  The
+// rounding to 2^-37 per sample is below the FP64 error of summing the samples
+// one by one into sums of order 1.
+constexpr double kStatScale = 68719476736.0;  // 2^36
+__device__ __forceinline__ unsigned long long stat_fx(double v) {
+    return (unsigned long long)__double2ll_rn(fmin(fmax(v, -128.0), 128.0) * kStatScale);
+}
+// one sample of node moments into a histogram slot: the 7 sums at h[q * stride],
+// the count at h[7 * stride] (kernels.cu:216-232)
+__device__ __forceinline__ void stat_sample(unsigned long long *h, size_t stride, double vy, double vr, double ph) {
+    atomicAdd(h + 0 * stride, stat_fx(vy));
+    atomicAdd(h + 1 * stride, stat_fx(vy * vy));
+    atomicAdd(h + 2 * stride, stat_fx(vr));
+    atomicAdd(h + 3 * stride, stat_fx(vr * vr));
+    atomicAdd(h + 4 * stride, stat_fx(vr * vy));
+    atomicAdd(h + 5 * stride, stat_fx(ph));
+    atomicAdd(h + 6 * stride, stat_fx(ph * ph));
+    atomicAdd(h + 7 * stride, 1ull);
+}
 
 // population row-block index of interior node (x,y,z): the 54 (FP32) or 27
 // (16-bit) direction rows of one (y,z) row lie next to each other, so one

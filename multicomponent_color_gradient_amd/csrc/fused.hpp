@@ -266,7 +266,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     constexpr int W = 66;
     constexpr int NT = 64 * kFusedWaves;
     __shared__ float phs[4][H][W];
-    extern __shared__ double hist[];  // STATS: [TY][NR][8], the last slot a u64 count
+    extern __shared__ unsigned long long hist[];  // STATS: [TY][NR][8], 7 fixed-point sums and the count
 
     const int t = xcd_tile(int(blockIdx.x), G.nwg);
     const int xt = t % G.nxt;
@@ -299,7 +299,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     const int gx = x0 - 1 + lc, gy = y0 - 1 + lr;
 
     if constexpr (STATS) {
-        for (int k = lid; k < TY * P.NR * 8; k += NT) hist[k] = 0.0;
+        for (int k = lid; k < TY * P.NR * 8; k += NT) hist[k] = 0ull;
     }
 
     // the lane's lead position after the optional periodic wrap, and whether
@@ -353,16 +353,8 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
                 if (bin >= 0) {
                     float ur = 0.0f;
                     if (rr > 1.0e-12f) ur = (m.ux * dx + m.uz * dz) / rr;
-                    const double vy = double(m.uy), vr = double(ur), ph = double(psi(m.rr, m.rb));
-                    double *hb = hist + (size_t(row) * P.NR + bin) * 8;
-                    atomicAdd(hb + 0, vy);
-                    atomicAdd(hb + 1, vy * vy);
-                    atomicAdd(hb + 2, vr);
-                    atomicAdd(hb + 3, vr * vr);
-                    atomicAdd(hb + 4, vr * vy);
-                    atomicAdd(hb + 5, ph);
-                    atomicAdd(hb + 6, ph * ph);
-                    atomicAdd(reinterpret_cast<unsigned long long *>(hb + 7), 1ull);
+                    stat_sample(hist + (size_t(row) * P.NR + bin) * 8, 1, double(m.uy), double(ur),
+                                double(psi(m.rr, m.rb)));
                 }
             }
         }
@@ -390,11 +382,11 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         for (int k = lid; k < TY * P.NR; k += NT) {
             const int row = k / P.NR, bin = k % P.NR;
             const int yy = y0 + row;
-            const double *hb = hist + size_t(k) * 8;
-            const unsigned long long c = reinterpret_cast<const unsigned long long *>(hb)[7];
+            const unsigned long long *hb = hist + size_t(k) * 8;
+            const unsigned long long c = hb[7];
             if (c == 0 || yy > P.NY - 2) continue;
             const size_t o = size_t(yy) * P.NR + bin;
-            for (int q = 0; q < 7; ++q) atomicAdd(&B.stats[q * nb + o], hb[q]);
+            for (int q = 0; q < 7; ++q) atomicAdd(&B.stats_fx[q * nb + o], hb[q]);
             atomicAdd(&B.count[o], c);
         }
     }
@@ -463,7 +455,7 @@ inline FusedGrid fused_grid(const DevParams &P, int num_cu) {
     return best_cost < 1e300 ? best : most;
 }
 
-inline size_t fused_stats_lds(const DevParams &P) { return size_t(kFusedTY) * P.NR * 8 * sizeof(double); }
+inline size_t fused_stats_lds(const DevParams &P) { return size_t(kFusedTY) * P.NR * 8 * sizeof(unsigned long long); }
 inline size_t fused_static_lds() { return size_t(4) * (kFusedTY + 2) * 66 * sizeof(float); }
 // the histogram fits next to the psi ring in the LDS a workgroup may use
 inline bool fused_has_stats(const DevParams &P, size_t lds_per_block) {

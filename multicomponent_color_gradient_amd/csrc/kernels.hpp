@@ -179,47 +179,36 @@ __global__ __launch_bounds__(256) void k_collide(DevParams P, DevBufs B) {
 // ---------------------------------------------------------------- statistics
 
 // Radial moments of u_y, u_r (and psi) per (y, r-bin), kernels.cu:188-233.
-// One block per (y, chunk of z): LDS privatised FP64 bins, then one global
-// FP64 atomic per touched bin and moment.
+// One block per (y, chunk of z): LDS privatised fixed-point bins (stat_sample),
+// then one global integer atomic per touched bin and moment into this step's
+// sums; k_stats_fold adds them to the FP64 sums.
 constexpr int kRadialZChunk = 8;
 
-__device__ __forceinline__ void radial_flush(const DevParams &P, int y, double *lsum, unsigned long long *lcnt,
-                                             double *stats, unsigned long long *count) {
+__device__ __forceinline__ void radial_flush(const DevParams &P, int y, const unsigned long long *lds,
+                                             unsigned long long *stats_fx, unsigned long long *count) {
     const size_t nb = size_t(P.NY) * P.NR;
     for (int b = threadIdx.x; b < P.NR; b += blockDim.x) {
-        const unsigned long long c = lcnt[b];
+        const unsigned long long c = lds[7 * P.NR + b];
         if (c == 0) continue;
         const size_t o = size_t(y) * P.NR + b;
-        for (int k = 0; k < 7; ++k) atomicAdd(&stats[k * nb + o], lsum[k * P.NR + b]);
+        for (int k = 0; k < 7; ++k) atomicAdd(&stats_fx[k * nb + o], lds[k * P.NR + b]);
         atomicAdd(&count[o], c);
     }
 }
 
-__device__ __forceinline__ void radial_add(const DevParams &P, int x, int z, const Moments &m, double *lsum,
-                                           unsigned long long *lcnt) {
+__device__ __forceinline__ void radial_add(const DevParams &P, int x, int z, const Moments &m, unsigned long long *lds) {
     float dx, dz, r;
     const int b = radial_bin(P, x, z, dx, dz, r);
     if (b < 0) return;
     float ur = 0.0f;
     if (r > 1.0e-12f) ur = (m.ux * dx + m.uz * dz) / r;
-    const double vy = double(m.uy), vr = double(ur);
-    const double ph = double(psi(m.rr, m.rb));
-    atomicAdd(&lsum[0 * P.NR + b], vy);
-    atomicAdd(&lsum[1 * P.NR + b], vy * vy);
-    atomicAdd(&lsum[2 * P.NR + b], vr);
-    atomicAdd(&lsum[3 * P.NR + b], vr * vr);
-    atomicAdd(&lsum[4 * P.NR + b], vr * vy);
-    atomicAdd(&lsum[5 * P.NR + b], ph);
-    atomicAdd(&lsum[6 * P.NR + b], ph * ph);
-    atomicAdd(&lcnt[b], 1ull);
+    stat_sample(lds + b, size_t(P.NR), double(m.uy), double(ur), double(psi(m.rr, m.rb)));
 }
 
-__global__ __launch_bounds__(256) void k_radial(DevParams P, const float *__restrict__ mom, double *stats,
-                                                unsigned long long *count) {
-    extern __shared__ double lds[];
-    double *lsum = lds;
-    unsigned long long *lcnt = reinterpret_cast<unsigned long long *>(lds + 7 * P.NR);
-    for (int b = threadIdx.x; b < 8 * P.NR; b += blockDim.x) lds[b] = 0.0;
+__global__ __launch_bounds__(256) void k_radial(DevParams P, const float *__restrict__ mom,
+                                                unsigned long long *stats_fx, unsigned long long *count) {
+    extern __shared__ unsigned long long lds[];  // [8][NR]: 7 fixed-point sums, then the counts
+    for (int b = threadIdx.x; b < 8 * P.NR; b += blockDim.x) lds[b] = 0ull;
     __syncthreads();
     const int y = 1 + int(blockIdx.x);
     const int z0 = 1 + int(blockIdx.y) * kRadialZChunk;
@@ -235,10 +224,26 @@ __global__ __launch_bounds__(256) void k_radial(DevParams P, const float *__rest
         m.ux = mom[2 * size_t(P.plane) + p];
         m.uy = mom[3 * size_t(P.plane) + p];
         m.uz = mom[4 * size_t(P.plane) + p];
-        radial_add(P, x, z, m, lsum, lcnt);
+        radial_add(P, x, z, m, lds);
     }
     __syncthreads();
-    radial_flush(P, y, lsum, lcnt, stats, count);
+    radial_flush(P, y, lds, stats_fx, count);
+}
+
+// Adds the fixed-point sums of one statistics step to the FP64 running sums and
+// clears them for the next step: one thread per (y, bin), so each FP64 sum
+// takes one add per step in step order and the sums are reproducible.
+__global__ __launch_bounds__(256) void k_stats_fold(size_t nb, unsigned long long *__restrict__ stats_fx,
+                                                    double *__restrict__ stats) {
+    const size_t o = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (o >= nb) return;
+    for (int q = 0; q < 7; ++q) {
+        const long long a = (long long)stats_fx[q * nb + o];
+        if (a != 0) {
+            stats[q * nb + o] += double(a) * (1.0 / kStatScale);
+            stats_fx[q * nb + o] = 0ull;
+        }
+    }
 }
 
 // ---------------------------------------------------------------- diagnostics
