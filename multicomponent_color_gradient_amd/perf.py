@@ -3,8 +3,9 @@
 Reads the files ``rocprofv3 --output-format csv`` writes:
   * ``*_kernel_stats.csv`` (``--kernel-trace --stats``): per-kernel call
     count and durations;
-  * ``*_counter_collection.csv`` (``--pmc FETCH_SIZE`` or ``--pmc WRITE_SIZE``):
-    one row per dispatch and counter, values in KiB;
+  * ``*_counter_collection.csv``: one row per dispatch and counter, from
+    ``--pmc`` of the sized L2 request counters (READ_COUNTERS, WRITE_COUNTERS:
+    exact bytes) or of FETCH_SIZE / WRITE_SIZE (KiB, scaled by a probe copy);
   * ``*_kernel_trace.csv`` (``--kernel-trace``): one row per dispatch with its
     grid, which separates the launches of one kernel on different grids.
 and turns them into the figures the benchmark reports: bytes per launch
@@ -15,6 +16,7 @@ utilities/countmlups.cuh:42-48 (steps * NX*NY*NZ / t / 1e6).
 Command line:
   python -m multicomponent_color_gradient_amd.perf stats <kernel_stats.csv>
   python -m multicomponent_color_gradient_amd.perf trace <kernel_trace.csv>
+  python -m multicomponent_color_gradient_amd.perf requests <read.csv> <write.csv> --kernel k_fused --nodes N
   python -m multicomponent_color_gradient_amd.perf traffic <fetch.csv> <write.csv> --kernel k_fused --nodes N
 """
 from __future__ import annotations
@@ -130,6 +132,62 @@ def traffic(fetch: dict, write: dict, kernel: str, nodes: int | None = None) -> 
     return out
 
 
+# Request-size counters of the L2 -> memory interface (one rocprofv3 --pmc pass
+# each for reads and writes, 4 and 2 TCC counters).  On gfx950 the derived
+# FETCH_SIZE expects 128-byte reads in TCC_BUBBLE, which stays 0, so it counts
+# every 128-byte read as 64 bytes; summing the sized requests needs no scale.
+READ_COUNTERS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum")
+WRITE_COUNTERS = ("TCC_EA0_WRREQ_64B_sum", "TCC_EA0_WRREQ_sum")
+
+
+def read_bytes(c: dict) -> float:
+    """Bytes read by one dispatch from its sized read requests."""
+    return 32.0 * c["TCC_EA0_RDREQ_32B_sum"] + 64.0 * c["TCC_EA0_RDREQ_64B_sum"] + 128.0 * c["TCC_EA0_RDREQ_128B_sum"]
+
+
+def write_bytes(c: dict) -> float:
+    """Bytes written by one dispatch: 64-byte requests, the rest 32-byte."""
+    n64 = c["TCC_EA0_WRREQ_64B_sum"]
+    return 64.0 * n64 + 32.0 * (c["TCC_EA0_WRREQ_sum"] - n64)
+
+
+def _dispatch_values(counters: dict, kernel: str, fn, need) -> list[float]:
+    vals = []
+    for name, disp in counters.items():
+        if kernel in name:
+            vals += [(d, fn(c)) for d, c in disp.items() if all(k in c for k in need)]
+    return [v for _, v in sorted(vals)]
+
+
+def probe_copy_check(rd: dict, wr: dict, nbytes: int = COPY_PROBE_BYTES, tol: float = 0.01) -> dict:
+    """The probe's copies of ``nbytes`` (``bench.py --copy-probe-mib``): the
+    dispatches of the copy kernel that read within ``tol`` of nbytes.  Each
+    must also write within ``tol`` of nbytes; returns their read/write ratios."""
+    reads = [b for b in _dispatch_values(rd, COPY_KERNEL, read_bytes, READ_COUNTERS[:3]) if abs(b / nbytes - 1) < tol]
+    writes = [b for b in _dispatch_values(wr, COPY_KERNEL, write_bytes, WRITE_COUNTERS) if abs(b / nbytes - 1) < tol]
+    if not reads or len(writes) != len(reads):
+        raise ValueError(f"probe copies of {nbytes} bytes not found: reads {reads}, writes {writes}")
+    return {"copies": len(reads), "read_ratio": sum(reads) / len(reads) / nbytes,
+            "write_ratio": sum(writes) / len(writes) / nbytes}
+
+
+def request_traffic(rd: dict, wr: dict, kernel: str, nodes: int | None = None) -> dict:
+    """Bytes per launch of ``kernel`` from the sized-request counters (a read
+    and a write collection), with the probe copies as the check of the counting."""
+    check = probe_copy_check(rd, wr)
+    r = _dispatch_values(rd, kernel, read_bytes, READ_COUNTERS[:3])
+    w = _dispatch_values(wr, kernel, write_bytes, WRITE_COUNTERS)
+    if not r or not w:
+        raise ValueError(f"no dispatches of {kernel}")
+    rb, wb = sum(r) / len(r), sum(w) / len(w)
+    out = {"kernel": kernel, "dispatches": len(r), "read_bytes": rb, "write_bytes": wb, "bytes_per_launch": rb + wb,
+           "probe_copy": check}
+    if nodes:
+        out["read_per_node"] = rb / nodes
+        out["write_per_node"] = wb / nodes
+    return out
+
+
 def stats_table(rows: list[dict]) -> str:
     lines = ["| kernel | calls | avg us | min us | max us | % |", "|---|---|---|---|---|---|"]
     for r in rows:
@@ -150,11 +208,20 @@ def main(argv=None) -> int:
     b.add_argument("write_csv")
     b.add_argument("--kernel", required=True)
     b.add_argument("--nodes", type=int, default=None)
+    q = sub.add_parser("requests")
+    q.add_argument("read_csv")
+    q.add_argument("write_csv")
+    q.add_argument("--kernel", required=True)
+    q.add_argument("--nodes", type=int, default=None)
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats_table(read_kernel_stats(args.csv)))
     elif args.cmd == "trace":
         print(trace_table(trace_stats(args.csv)))
+    elif args.cmd == "requests":
+        t = request_traffic(read_counters(args.read_csv), read_counters(args.write_csv), args.kernel, args.nodes)
+        for k, v in t.items():
+            print(f"{k}: {v}")
     else:
         t = traffic(read_counters(args.fetch_csv), read_counters(args.write_csv), args.kernel, args.nodes)
         for k, v in t.items():

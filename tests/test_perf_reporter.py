@@ -103,3 +103,47 @@ def test_trace_groups_launches_by_grid(tmp_path):
     assert [(r["name"], r["grid"], r["calls"]) for r in rows] == [("k", 32768, 2), ("k", 512, 1), ("q", 512, 1)]
     assert rows[0]["avg_ns"] == 2000 and rows[0]["min_ns"] == 1000 and rows[0]["max_ns"] == 3000
     assert "| `k` | 32768 | 2 | 2.0 | 1.0 | 3.0 |" in perf.trace_table(rows)
+
+
+def sized(tmp_path):
+    # setup copy, three probe copies of 1 GiB (128-byte reads, 64-byte writes), two step dispatches
+    gib = 1 << 30
+    rd, wr = [], []
+    for disp, name, n32, n64, n128, w64, w32 in [
+            (1, "__amd_rocclr_copyBuffer", 0, 24, 1, 30, 2),
+            (2, "void mcg::k_fused<false, 0>(...)", 10, 300, 74_000_000, 112_000_000, 7000),
+            (3, "void mcg::k_fused<false, 0>(...)", 10, 340, 74_100_000, 112_100_000, 6000),
+            (4, "__amd_rocclr_copyBuffer", 0, 200, gib // 128, gib // 64, 0),
+            (5, "__amd_rocclr_copyBuffer", 0, 202, gib // 128, gib // 64, 0),
+            (6, "__amd_rocclr_copyBuffer", 0, 318, gib // 128, gib // 64, 0)]:
+        rd += [(disp, name, "TCC_EA0_RDREQ_32B_sum", n32), (disp, name, "TCC_EA0_RDREQ_64B_sum", n64),
+               (disp, name, "TCC_EA0_RDREQ_128B_sum", n128), (disp, name, "TCC_EA0_RDREQ_sum", n32 + n64 + n128)]
+        wr += [(disp, name, "TCC_EA0_WRREQ_64B_sum", w64), (disp, name, "TCC_EA0_WRREQ_sum", w64 + w32)]
+    rp, wp = tmp_path / "rd.csv", tmp_path / "wr.csv"
+    write_counters(rp, rd)
+    write_counters(wp, wr)
+    return str(rp), str(wp)
+
+
+def test_request_size_traffic(tmp_path):
+    rp, wp = sized(tmp_path)
+    t = perf.request_traffic(perf.read_counters(rp), perf.read_counters(wp), "k_fused", nodes=100)
+    assert t["dispatches"] == 2
+    r2 = 32 * 10 + 64 * 300 + 128 * 74_000_000
+    r3 = 32 * 10 + 64 * 340 + 128 * 74_100_000
+    assert t["read_bytes"] == pytest.approx((r2 + r3) / 2)
+    assert t["write_bytes"] == pytest.approx((64 * 112_000_000 + 32 * 7000 + 64 * 112_100_000 + 32 * 6000) / 2)
+    assert t["probe_copy"]["copies"] == 3
+    # the 64-byte reads of the probe copies are the only excess over 1 GiB
+    assert t["probe_copy"]["read_ratio"] == pytest.approx(1 + 64 * 240 / (1 << 30))
+    assert t["probe_copy"]["write_ratio"] == pytest.approx(1.0)
+    assert t["read_per_node"] == pytest.approx(t["read_bytes"] / 100)
+
+
+def test_request_traffic_needs_the_probe(tmp_path):
+    rp, wp = sized(tmp_path)
+    rows = [r for r in csv.DictReader(open(rp)) if r["Dispatch_Id"] not in ("4", "5", "6")]
+    write_counters(rp, [(int(r["Dispatch_Id"]), r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"]))
+                        for r in rows])
+    with pytest.raises(ValueError):
+        perf.request_traffic(perf.read_counters(rp), perf.read_counters(wp), "k_fused")
