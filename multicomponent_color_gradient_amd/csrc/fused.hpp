@@ -215,23 +215,33 @@ __device__ __forceinline__ float raw_psi(const RawGather<ST> &r) {
 }
 
 struct FusedGrid {
+    int ty;  // core rows per tile of the instance the grid was made for
     int nxt, nyb, nzs, zlen, nwg;
 };
 
-#ifndef MCG_FUSED_WAVES_PER_EU
-#define MCG_FUSED_WAVES_PER_EU 3
-#endif
-// Waves: MCG_FUSED_TY core waves (one tile row each) followed by 3 halo waves.
-// The hardware deals the waves of a workgroup to the 4 SIMDs of a CU in turn
-// (wave w on SIMD w mod 4, read from the HW_ID register on MI355X), so with 8
-// core waves every SIMD runs 2 core waves (and 3 of them one halo wave); with 9
-// one SIMD would run 3 core waves, and the 16-bit step, whose arithmetic
-// nearly fills the SIMDs, ran 7% slower at 256x512x256.
-#ifndef MCG_FUSED_TY
-#define MCG_FUSED_TY 8
-#endif
-constexpr int kFusedTY = MCG_FUSED_TY;
-constexpr int kFusedWaves = kFusedTY + 3;
+// Tile shape of a k_fused instance: TY core waves (one y-row of 64 x-columns
+// each) followed by 3 halo waves, and where the moments of the planes in
+// flight live.  The hardware deals the waves of a workgroup to the 4 SIMDs of
+// a CU in turn (wave w on SIMD w mod 4, read from the HW_ID register on
+// MI355X), so TY a multiple of 4 gives every SIMD the same number of core
+// waves; with 9 core rows one SIMD ran 3 of them and the 16-bit step, whose
+// arithmetic nearly fills the SIMDs, was 7% slower at 256x512x256.
+//   16-bit storage, plain steps: 12 core rows (15 waves, 4 per SIMD, at most
+//     128 VGPRs): the halo ring is 156 positions for 768 nodes instead of 148
+//     for 512, and every SIMD runs 3 core waves.
+//   FP32 (54 words in flight per lane) and the statistics steps: 8 core rows
+//     (11 waves, 3 per SIMD).
+// kMomLds: the moments of planes z and z+1 wait in an LDS ring instead of
+// registers (22 VGPRs fewer, no register rotation per plane); the statistics
+// steps keep them in registers and give the LDS to the histogram.
+template <int ST, bool STATS>
+struct FusedShape {
+    static constexpr bool k16 = ST != MCG_STORE_FP32;
+    static constexpr int TY = (k16 && !STATS) ? 12 : 8;
+    static constexpr int kWaves = TY + 3;
+    static constexpr int kWavesPerEU = (k16 && !STATS) ? 4 : 3;
+    static constexpr bool kMomLds = !STATS;
+};
 
 // Tile of workgroup b when the hardware deals workgroups round-robin over the
 // 8 XCDs: XCD k gets a contiguous range of tiles, so tiles that share halo
@@ -241,31 +251,36 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return k * q + min(k, r) + (b >> 3);
 }
 
-// Workgroup = TY core waves (one y-row of 64 x-columns each) + halo waves.
 // Every lane owns one gather position of the (TY+2) x 66 psi tile: core lanes
 // their node, the first halo wave the row y0-1, the second the row y0+TY, the
-// others the columns x0-1 and x0+64 (lanes < TY+2).  Iteration zi
-// gathers the lead plane zi+1 into the psi ring (core lanes keep the moments
-// in registers), passes one barrier, and collides plane zi; the first two
-// iterations only fill the ring and the moments of plane zb.  The gather and
-// the collision each appear once in the code, which keeps the loop inside the
+// third the columns x0-1 and x0+64 (lanes < 2 (TY+2)).  Iteration zi
+// gathers the lead plane zi+2 into the psi ring (core lanes keep its moments),
+// passes one barrier, and collides plane zi; the first three iterations only
+// fill the ring and the moments of planes zb and zb+1.  The gather and the
+// collision each appear once in the code, which keeps the loop inside the
 // instruction cache.
 // MOM: the core lanes also store the moments of their node (B.mom_out), the
 // variant the adjoint's recomputation runs.  EXT: x/z walls or the body force
 // are on; without them the kernel is compiled with both known to be off.
 template <bool STATS, int ST, bool MOM = false, bool EXT = false>
-__global__ __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_per_eu(MCG_FUSED_WAVES_PER_EU)))
+__global__ __launch_bounds__((64 * FusedShape<ST, STATS>::kWaves))
+__attribute__((amdgpu_waves_per_eu(FusedShape<ST, STATS>::kWavesPerEU)))
 void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     if constexpr (!EXT) {
         P.wall = 0;
         P.grav = 0.0f;
         P.ghalf = 0.0f;
     }
-    constexpr int TY = kFusedTY;
+    using Shape = FusedShape<ST, STATS>;
+    constexpr int TY = Shape::TY;
+    constexpr bool MLDS = Shape::kMomLds;
     constexpr int H = TY + 2;
     constexpr int W = 66;
-    constexpr int NT = 64 * kFusedWaves;
+    constexpr int NT = 64 * Shape::kWaves;
+    constexpr int NMOM = 11;
     __shared__ float phs[4][H][W];
+    // moment ring: [plane mod 3][row][moment][x], one 256-byte row per moment
+    __shared__ float mring[MLDS ? 3 * TY * NMOM * 64 : 1];
     extern __shared__ unsigned long long hist[];  // STATS: [TY][NR][8], 7 fixed-point sums and the count
 
     const int t = xcd_tile(int(blockIdx.x), G.nwg);
@@ -309,8 +324,11 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     bool on_grid = gx >= 0 && gx < P.NX && gy >= 0 && gy < P.NY;
     if (on_grid) grad_neighbor(P, gx, gy, 1, gxn, gyn, gzn_unused);
     const bool split_xy = active && on_grid && gxn >= 1 && gxn <= P.NX - 2 && gyn >= 1 && gyn <= P.NY - 2 &&
-                          !(P.y_mode == MCG_Y_JET && (gyn == 1 || gyn == P.NY - 2)) &&
-                          !(P.wall && (gxn == 1 || gxn == P.NX - 2));
+                          !(P.wall && (gxn == 1 || gxn == P.NX - 2)) &&
+                          !(P.y_mode == MCG_Y_JET && (gyn == 1 || gyn == P.NY - 2));
+
+    // the moments ring slot of plane q: mring[((q mod 3) * TY + row) * NMOM + k][tx]
+    auto mslot = [&](int q) -> float * { return mring + (size_t((q % 3) * TY + row) * NMOM) * 64 + tx; };
 
     Moments cur{}, nx1{}, nx2{};
     RawGather<ST> raw;
@@ -328,6 +346,11 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         const bool split = lead && split_xy && zln >= 1 && zln <= P.NZ - 2 && !(P.wall && (zln == 1 || zln == P.NZ - 2));
         if (split) raw_issue<ST>(P, B, gxn, gyn, zln, raw);
         if (node && z >= zb) {
+            if constexpr (MLDS) {
+                const float *ms = mslot(z);
+                cur = Moments{ms[0 * 64], ms[1 * 64], ms[2 * 64], ms[3 * 64], ms[4 * 64], ms[5 * 64],
+                              ms[6 * 64], ms[7 * 64], ms[8 * 64], ms[9 * 64], ms[10 * 64]};
+            }
             const Moments &m = cur;
             auto phi = [&](auto Ic) -> float {
                 constexpr int I = decltype(Ic)::value;
@@ -369,11 +392,21 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
                 bool has_m;
                 ph = gather_pos<ST>(P, B, gx, gy, zl, nx2, has_m);
             }
+            if constexpr (MLDS) {
+                if (core) {
+                    float *ms = mslot(zl);
+                    const float v[NMOM] = {nx2.rr, nx2.rb, nx2.ux, nx2.uy, nx2.uz, nx2.pxx,
+                                           nx2.pxy, nx2.pyy, nx2.pyz, nx2.pzz, nx2.pxz};
+                    for (int k = 0; k < NMOM; ++k) ms[k * 64] = v[k];
+                }
+            }
             phs[zl & 3][lr][lc] = ph;
         }
         __syncthreads();
-        cur = nx1;
-        nx1 = nx2;
+        if constexpr (!MLDS) {
+            cur = nx1;
+            nx1 = nx2;
+        }
     }
 
     if constexpr (STATS) {
@@ -392,36 +425,40 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     }
 }
 
-// resident workgroups per CU under the register budget of MCG_FUSED_WAVES_PER_EU
-constexpr int fused_wg_per_cu() {
-    return (4 * MCG_FUSED_WAVES_PER_EU) / kFusedWaves > 0 ? (4 * MCG_FUSED_WAVES_PER_EU) / kFusedWaves : 1;
+// core rows of the plain (no statistics) step of a storage: the shape the
+// launch-shape timing and the segment counts refer to
+inline int fused_plain_ty(int storage) {
+    return storage == MCG_STORE_FP32 ? FusedShape<MCG_STORE_FP32, false>::TY : FusedShape<MCG_STORE_FP16, false>::TY;
 }
 
-// The launch of zs z segments: tiles of 64 x-columns by kFusedTY rows, each
+// The launch of zs z segments: tiles of 64 x-columns by ty rows, each
 // marching through a segment of ceil(IZ/zs) planes.
-inline FusedGrid fused_grid_for(const DevParams &P, int zs) {
+inline FusedGrid fused_grid_for(const DevParams &P, int zs, int ty) {
     FusedGrid G;
+    G.ty = ty;
     G.nxt = P.PX / 64;
-    G.nyb = (P.IY + kFusedTY - 1) / kFusedTY;
+    G.nyb = (P.IY + ty - 1) / ty;
     G.zlen = (P.IZ + zs - 1) / zs;
     G.nzs = (P.IZ + G.zlen - 1) / G.zlen;
     G.nwg = G.nxt * G.nyb * G.nzs;
     return G;
 }
 
-// Segment counts worth trying: every distinct segmentation with segments of
-// at least 8 planes (at most 16 of them, spread over the range).  Each
-// segment costs 2 fill planes of gathers, and the best count depends on how
-// the concurrently running tiles' rows fall on the memory channels: at
-// 256x512x256 the 16-bit step takes 2.14 ms with 17 segments, 2.27 ms with 12
-// or 16 and 3.06 ms with one (all workgroups loading the same planes at the
-// same time).  The solver times the candidates on its first steps and keeps
-// the fastest; the results do not depend on the segmentation.
+// Segment counts worth trying for the plain step: every distinct segmentation
+// with segments of at least 8 planes (at most 16 of them, spread over the
+// range).  Each segment costs 2 fill planes of gathers, and the best count
+// depends on how the concurrently running tiles' rows fall on the memory
+// channels: at 256x512x256 the 16-bit step (8-row tiles) took 2.14 ms with 17
+// segments, 2.27 ms with 12 or 16 and 3.06 ms with one (all workgroups loading
+// the same planes at the same time).  The solver times the candidates on its
+// first steps and keeps the fastest; the results do not depend on the
+// segmentation.
 inline std::vector<FusedGrid> fused_candidates(const DevParams &P) {
+    const int ty = fused_plain_ty(P.storage);
     std::vector<FusedGrid> all;
     const int maxseg = std::max(1, std::min(64, P.IZ / 8));
     for (int zs = 1; zs <= maxseg; ++zs) {
-        const FusedGrid G = fused_grid_for(P, zs);
+        const FusedGrid G = fused_grid_for(P, zs, ty);
         if (all.empty() || all.back().nzs != G.nzs) all.push_back(G);
     }
     if (all.size() <= 16) return all;
@@ -432,17 +469,14 @@ inline std::vector<FusedGrid> fused_candidates(const DevParams &P) {
 
 // The segmentation used before (or without) the timing: among the counts that
 // give at least kMinRounds rounds of resident workgroups (4 for FP32, 16 for
-// 16-bit storage), the fewest rounds x (segment length + 2 fill planes);
-// grids too small for that many rounds take the most segments.
+// 16-bit storage; one workgroup per CU), the fewest rounds x (segment length +
+// 2 fill planes); grids too small for that many rounds take the most segments.
 inline FusedGrid fused_grid(const DevParams &P, int num_cu) {
     const int min_rounds = P.storage == MCG_STORE_FP32 ? 4 : 16;
-    const int slots = std::max(1, num_cu * fused_wg_per_cu());
+    const int slots = std::max(1, num_cu);
     FusedGrid best{}, most{};
     double best_cost = 1e300;
     for (const FusedGrid &G : fused_candidates(P)) {
-#ifdef MCG_FUSED_ZSEG
-        if (G.nzs != fused_grid_for(P, MCG_FUSED_ZSEG).nzs) continue;  // a fixed segment count (tuning builds)
-#endif
         most = G;
         const int rounds = (G.nwg + slots - 1) / slots;
         if (rounds < min_rounds) continue;
@@ -455,30 +489,41 @@ inline FusedGrid fused_grid(const DevParams &P, int num_cu) {
     return best_cost < 1e300 ? best : most;
 }
 
-inline size_t fused_stats_lds(const DevParams &P) { return size_t(kFusedTY) * P.NR * 8 * sizeof(unsigned long long); }
-inline size_t fused_static_lds() { return size_t(4) * (kFusedTY + 2) * 66 * sizeof(float); }
+// the statistics instance (8-row tiles, moments in registers) keeps its
+// histogram in dynamic LDS next to the psi ring
+constexpr int kStatsTY = FusedShape<MCG_STORE_FP32, true>::TY;
+static_assert(FusedShape<MCG_STORE_FP16, true>::TY == kStatsTY, "one statistics tile height");
+inline size_t fused_stats_lds(const DevParams &P) { return size_t(kStatsTY) * P.NR * 8 * sizeof(unsigned long long); }
+inline size_t fused_static_lds() { return size_t(4) * (kStatsTY + 2) * 66 * sizeof(float); }
 // the histogram fits next to the psi ring in the LDS a workgroup may use
 inline bool fused_has_stats(const DevParams &P, size_t lds_per_block) {
     return fused_stats_lds(P) + fused_static_lds() <= lds_per_block;
 }
 
+// one instance on grid G, re-tiled to the instance's rows with G's segment count
+template <bool STATS, int ST, bool MOM, bool EXT>
+inline void launch_fused_one(const DevParams &P, const DevBufs &B, hipStream_t stream, const FusedGrid &G, size_t lds) {
+    using Shape = FusedShape<ST, STATS>;
+    const FusedGrid Gi = G.ty == Shape::TY ? G : fused_grid_for(P, G.nzs, Shape::TY);
+    hipLaunchKernelGGL((k_fused<STATS, ST, MOM, EXT>), dim3(Gi.nwg), dim3(64, Shape::kWaves, 1), lds, stream, P, B, Gi);
+}
+
 template <int ST>
 inline int launch_fused_st(const DevParams &P, const DevBufs &B, hipStream_t stream, const FusedGrid &G) {
-    const dim3 block(64, kFusedWaves, 1);
     if (P.wall || P.grav != 0.0f) {
         if (B.do_stats) {
-            hipLaunchKernelGGL((k_fused<true, ST, false, true>), dim3(G.nwg), block, fused_stats_lds(P), stream, P, B, G);
+            launch_fused_one<true, ST, false, true>(P, B, stream, G, fused_stats_lds(P));
         } else if (B.mom_out) {
-            hipLaunchKernelGGL((k_fused<false, ST, true, true>), dim3(G.nwg), block, 0, stream, P, B, G);
+            launch_fused_one<false, ST, true, true>(P, B, stream, G, 0);
         } else {
-            hipLaunchKernelGGL((k_fused<false, ST, false, true>), dim3(G.nwg), block, 0, stream, P, B, G);
+            launch_fused_one<false, ST, false, true>(P, B, stream, G, 0);
         }
     } else if (B.do_stats) {
-        hipLaunchKernelGGL((k_fused<true, ST>), dim3(G.nwg), block, fused_stats_lds(P), stream, P, B, G);
+        launch_fused_one<true, ST, false, false>(P, B, stream, G, fused_stats_lds(P));
     } else if (B.mom_out) {
-        hipLaunchKernelGGL((k_fused<false, ST, true>), dim3(G.nwg), block, 0, stream, P, B, G);
+        launch_fused_one<false, ST, true, false>(P, B, stream, G, 0);
     } else {
-        hipLaunchKernelGGL((k_fused<false, ST>), dim3(G.nwg), block, 0, stream, P, B, G);
+        launch_fused_one<false, ST, false, false>(P, B, stream, G, 0);
     }
     return int(hipGetLastError());
 }
