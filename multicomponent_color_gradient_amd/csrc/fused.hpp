@@ -445,14 +445,14 @@ inline FusedGrid fused_grid_for(const DevParams &P, int zs, int ty) {
 }
 
 // Segment counts worth trying for the plain step: every distinct segmentation
-// with segments of at least 8 planes (at most 16 of them, spread over the
-// range).  Each segment costs 2 fill planes of gathers, and the best count
-// depends on how the concurrently running tiles' rows fall on the memory
-// channels: at 256x512x256 the 16-bit step (8-row tiles) took 2.14 ms with 17
-// segments, 2.27 ms with 12 or 16 and 3.06 ms with one (all workgroups loading
-// the same planes at the same time).  The solver times the candidates on its
-// first steps and keeps the fastest; the results do not depend on the
-// segmentation.
+// with segments of at least 8 planes (at most 64 segments).  Each segment costs
+// 2 fill planes of gathers, and the best count depends on which tiles run at
+// the same time: at 256x512x256 the 16-bit step (12-row tiles) takes 1.96 ms
+// with 15 segments, 2.00 with 7, 9 or 13, 2.10 with 16 and 2.36 with 6, and
+// the FP32 step 3.25 ms with 4, 3.30 with 5 and 3.60 with 29, whatever the
+// padding of the row blocks (32 to 320 words).  The solver times every
+// candidate on its first steps and keeps the fastest; the results do not
+// depend on the segmentation.
 inline std::vector<FusedGrid> fused_candidates(const DevParams &P) {
     const int ty = fused_plain_ty(P.storage);
     std::vector<FusedGrid> all;
@@ -461,10 +461,7 @@ inline std::vector<FusedGrid> fused_candidates(const DevParams &P) {
         const FusedGrid G = fused_grid_for(P, zs, ty);
         if (all.empty() || all.back().nzs != G.nzs) all.push_back(G);
     }
-    if (all.size() <= 16) return all;
-    std::vector<FusedGrid> pick;
-    for (int k = 0; k < 16; ++k) pick.push_back(all[size_t(k) * (all.size() - 1) / 15]);
-    return pick;
+    return all;
 }
 
 // The segmentation used before (or without) the timing: among the counts that
