@@ -160,7 +160,8 @@ __device__ __forceinline__ float gather_pos(const DevParams &P0, const DevBufs &
 
 // Split gather for software pipelining: raw_issue starts the 27 or 54 loads
 // of an interior node whose sources are all population rows (no inlet or
-// outlet source buffer), raw_reduce turns the words into moments later.  The
+// outlet source buffer; a wall source is the node's own opposite row),
+// raw_reduce turns the words into moments later.  The
 // decode and the summation order are those of fast_moments, so the moments
 // are bitwise the same.
 template <int ST>
@@ -182,6 +183,16 @@ __device__ __forceinline__ void raw_issue(const DevParams &P0, const DevBufs &B,
         if constexpr (cy < 0) off += s.wym;
         if constexpr (cz > 0) off += s.wzp;
         if constexpr (cz < 0) off += s.wzm;
+        if constexpr (cx != 0 || cz != 0) {
+            // half-way bounce-back at an x/z wall: the node's own word of the
+            // opposite direction (row opp(I), one PX away), as fast_load reads it
+            bool bb = false;
+            if constexpr (cx > 0) bb = bb || s.bxp;
+            if constexpr (cx < 0) bb = bb || s.bxm;
+            if constexpr (cz > 0) bb = bb || s.bzp;
+            if constexpr (cz < 0) bb = bb || s.bzm;
+            if (bb) off = s.p + uint32_t(opp(I) - I) * uint32_t(P.PX);
+        }
         Store<ST>::template load_raw<I>(B.f_src, uint32_t(P.PX), off, r.w + I * Store<ST>::kWords);
     });
 }
@@ -324,7 +335,6 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     bool on_grid = gx >= 0 && gx < P.NX && gy >= 0 && gy < P.NY;
     if (on_grid) grad_neighbor(P, gx, gy, 1, gxn, gyn, gzn_unused);
     const bool split_xy = active && on_grid && gxn >= 1 && gxn <= P.NX - 2 && gyn >= 1 && gyn <= P.NY - 2 &&
-                          !(P.wall && (gxn == 1 || gxn == P.NX - 2)) &&
                           !(P.y_mode == MCG_Y_JET && (gyn == 1 || gyn == P.NY - 2));
 
     // the moments ring slot of plane q: mring[((q mod 3) * TY + row) * NMOM + k][tx]
@@ -343,7 +353,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         const bool lead = zl <= ze + 1;
         int zln = zl;
         if (P.wrap_gradient && !P.wall) zln = wrapN(zl, P.NZ);
-        const bool split = lead && split_xy && zln >= 1 && zln <= P.NZ - 2 && !(P.wall && (zln == 1 || zln == P.NZ - 2));
+        const bool split = lead && split_xy && zln >= 1 && zln <= P.NZ - 2;
         if (split) raw_issue<ST>(P, B, gxn, gyn, zln, raw);
         if (node && z >= zb) {
             if constexpr (MLDS) {
