@@ -82,11 +82,15 @@ def trace_table(rows: list[dict]) -> str:
     return "\n".join(lines)
 
 
-def read_counters(path: str) -> dict:
-    """{kernel name: {dispatch id: {counter: value summed over the rows of that dispatch}}}."""
+def read_counters(path: str, grid: int | None = None) -> dict:
+    """{kernel name: {dispatch id: {counter: value summed over the rows of that dispatch}}},
+    optionally only the dispatches on a grid of ``grid`` work-items (one workload of a
+    run that launches the same kernel on several grids)."""
     out: dict = defaultdict(lambda: defaultdict(lambda: defaultdict(float)))
     with open(path, newline="") as fh:
         for r in csv.DictReader(fh):
+            if grid is not None and int(r["Grid_Size"]) != grid and not r["Kernel_Name"].startswith(COPY_KERNEL):
+                continue
             out[r["Kernel_Name"]][int(r["Dispatch_Id"])][r["Counter_Name"]] += float(r["Counter_Value"])
     return out
 
@@ -213,13 +217,15 @@ def main(argv=None) -> int:
     q.add_argument("write_csv")
     q.add_argument("--kernel", required=True)
     q.add_argument("--nodes", type=int, default=None)
+    q.add_argument("--grid", type=int, default=None, help="only dispatches on a grid of this many work-items")
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats_table(read_kernel_stats(args.csv)))
     elif args.cmd == "trace":
         print(trace_table(trace_stats(args.csv)))
     elif args.cmd == "requests":
-        t = request_traffic(read_counters(args.read_csv), read_counters(args.write_csv), args.kernel, args.nodes)
+        t = request_traffic(read_counters(args.read_csv, args.grid), read_counters(args.write_csv, args.grid),
+                            args.kernel, args.nodes)
         for k, v in t.items():
             print(f"{k}: {v}")
     else:

@@ -147,3 +147,17 @@ def test_request_traffic_needs_the_probe(tmp_path):
                         for r in rows])
     with pytest.raises(ValueError):
         perf.request_traffic(perf.read_counters(rp), perf.read_counters(wp), "k_fused")
+
+
+def test_counters_of_one_grid(tmp_path):
+    p = tmp_path / "c.csv"
+    with open(p, "w", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow(COUNTER_COLS)
+        for disp, grid, name, value in [(1, 1024, "k", 5.0), (2, 2048, "k", 7.0), (3, 64, "__amd_rocclr_copyBuffer", 1.0)]:
+            w.writerow([disp, disp, "Agent 1", 1, 1, 1, grid, 1, name, 256, 0, 0, 32, 0, 32, "X", value, 0, 1])
+    c = perf.read_counters(str(p), grid=2048)
+    assert perf.per_dispatch(c, "k", "X") == [7.0]
+    # the probe's copies are kept whatever their grid
+    assert perf.per_dispatch(c, "copyBuffer", "X") == [1.0]
+    assert perf.per_dispatch(perf.read_counters(str(p)), "k", "X") == [5.0, 7.0]
