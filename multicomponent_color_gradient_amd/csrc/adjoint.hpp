@@ -28,9 +28,9 @@ __device__ __forceinline__ float mom_psi(const DevParams &P, const float *__rest
     return psi(mom[o], mom[size_t(P.plane) + o]);
 }
 
-// node-layout row of direction i (colour c) at interior node (x,y,z)
-__device__ __forceinline__ uint32_t nrow(const DevParams &P, int c, int i, int x, int y, int z) {
-    return uint32_t(c * Q + i) * uint32_t(P.PX) + pidx(P, x, y, z);
+// node-layout element of direction i (colour c) at interior node (x,y,z)
+__device__ __forceinline__ size_t nrow(const DevParams &P, int c, int i, int x, int y, int z) {
+    return size_t(uint32_t(c * Q + i) * uint32_t(P.PX)) + pidx64(P, x, y, z);
 }
 
 // wave sum of a double (64 lanes)
@@ -128,10 +128,10 @@ void k_adj_node(DevParams P, const float *__restrict__ mom, const float *__restr
         ext.taub = P.taub;
         ext.grav = P.grav;
         const AdjNode<float> n = adj_setup<float>(m, Fx, Fy, Fz, A_node(P, m[0], m[1]), P.beta, omega, ext);
-        const uint32_t self = pidx(P, x, y, z);
+        const size_t self = pidx64(P, x, y, z);
         // mu_i(x) = lambda_i(x + c_i), loaded where the pair needs it
         const AdjOut<float> a = adj_vjp_fn<float>(n, [&](int i, float &r, float &b) {
-            uint32_t q;
+            size_t q;
             int row = i;
             if constexpr (MODE == kAdjJet) {
                 const int ty = y + kCY[i];
@@ -139,20 +139,20 @@ void k_adj_node(DevParams P, const float *__restrict__ mom, const float *__restr
                     r = b = 0.0f;
                     return;
                 }
-                q = pidx(P, wrapN(x + kCX[i], P.NX), ty, wrapN(z + kCZ[i], P.NZ));
+                q = pidx64(P, wrapN(x + kCX[i], P.NX), ty, wrapN(z + kCZ[i], P.NZ));
             } else if constexpr (MODE == kAdjWall) {
                 const int tx = x + kCX[i], tz = z + kCZ[i];
                 if (tx < 1 || tx > P.NX - 2 || tz < 1 || tz > P.NZ - 2) {  // bounced back into direction opp(i)
                     q = self;
                     row = opp(i);
                 } else {
-                    q = pidx(P, tx, wrapN(y + kCY[i], P.NY), tz);
+                    q = pidx64(P, tx, wrapN(y + kCY[i], P.NY), tz);
                 }
             } else {
-                q = pidx(P, wrapN(x + kCX[i], P.NX), wrapN(y + kCY[i], P.NY), wrapN(z + kCZ[i], P.NZ));
+                q = pidx64(P, wrapN(x + kCX[i], P.NX), wrapN(y + kCY[i], P.NY), wrapN(z + kCZ[i], P.NZ));
             }
-            r = lam_in[uint32_t(row) * uint32_t(P.PX) + q];
-            b = lam_in[uint32_t(Q + row) * uint32_t(P.PX) + q];
+            r = lam_in[size_t(uint32_t(row) * uint32_t(P.PX)) + q];
+            b = lam_in[size_t(uint32_t(Q + row) * uint32_t(P.PX)) + q];
         });
         float lm[11];
         for (int k = 0; k < 11; ++k) lm[k] = a.m[k];

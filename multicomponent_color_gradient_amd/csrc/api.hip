@@ -272,6 +272,7 @@ void fill_params(const mcg_case &c, int storage, DevParams &P) {
     P.IZ = c.NZ - 2;
     P.plane = uint32_t(size_t(P.PX) * P.IY * P.IZ);
     P.rowstride = uint32_t(P.PX * (storage == MCG_STORE_FP32 ? 2 * Q : Q) + kRowPad);
+    P.planeel = P.rowstride * uint32_t(P.IY);
     P.y_mode = c.y_mode;
     P.wrap_gradient = c.wrap_gradient;
     P.visc_mode = c.viscosity_mode;
@@ -552,8 +553,12 @@ int mcg_case_validate(const mcg_case *c) {
     // the 2^20 the fixed-point statistics sums hold)
     if (c->NX > 2048) return fail(MCG_E_INVALID, "NX > 2048 (radial bins NX/2 must fit the LDS histogram)");
     const size_t px = size_t((c->NX - 2 + 63) / 64 * 64);
-    if ((px * 54 + kRowPad) * size_t(c->NY - 2) * size_t(c->NZ - 2) >= (size_t(1) << 32))
-        return fail(MCG_E_INVALID, "grid too large for 32-bit population offsets");
+    // population addresses are a z-plane base plus a 32-bit byte offset within
+    // the plane, moments and node-layout fields 32-bit element offsets
+    if ((px * 54 + kRowPad) * size_t(c->NY - 2) * sizeof(float) >= (size_t(1) << 32))
+        return fail(MCG_E_INVALID, "grid too large: one z-plane of populations must stay below 4 GiB");
+    if (px * size_t(c->NY - 2) * size_t(c->NZ - 2) >= (size_t(1) << 32))
+        return fail(MCG_E_INVALID, "grid too large: more than 2^32 padded interior nodes");
     if (c->y_mode != MCG_Y_JET && c->y_mode != MCG_Y_PERIODIC) return fail(MCG_E_INVALID, "unknown y_mode");
     if (c->viscosity_mode != MCG_VISC_SINGLE && c->viscosity_mode != MCG_VISC_INTERFACE)
         return fail(MCG_E_INVALID, "unknown viscosity_mode");
@@ -584,10 +589,6 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
     s->storage = storage;
     s->stream = static_cast<hipStream_t>(stream);
     fill_params(*c, storage, s->P);
-    if (storage != MCG_STORE_FP32 && s->pop_bytes() >= (size_t(1) << 32)) {
-        delete s;
-        return fail(MCG_E_INVALID, "16-bit population storage is limited to 4 GiB per buffer (32-bit byte offsets)");
-    }
     if (hipGetDevice(&s->device) != hipSuccess) {
         delete s;
         return fail(MCG_E_HIP, "hipGetDevice failed (no GPU?)");

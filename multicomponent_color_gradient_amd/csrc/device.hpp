@@ -42,6 +42,7 @@ struct DevParams {
     int IY, IZ;       // NY-2, NZ-2
     uint32_t plane;   // PX*IY*IZ, elements of one moment plane
     uint32_t rowstride;  // populations: elements of one (y,z) row block, PX * (54 or 27)
+    uint32_t planeel;    // populations: elements of one z-plane, rowstride * IY (its bytes fit 32 bits)
     int y_mode, wrap_gradient, visc_mode;
     int sponge_ystart, sponge_yend;
     float nu, sponge_K;
@@ -107,11 +108,30 @@ __device__ __forceinline__ void stat_sample(unsigned long long *h, size_t stride
     atomicAdd(h + 7 * stride, 1ull);
 }
 
-// population row-block index of interior node (x,y,z): the 54 (FP32) or 27
-// (16-bit) direction rows of one (y,z) row lie next to each other, so one
-// wavefront's gather touches a few contiguous 1 KB rows instead of 54 planes
+// Population arrays: the 54 (FP32) or 27 (16-bit) direction rows of one (y,z)
+// row lie next to each other, so one wavefront's gather touches a few
+// contiguous 1 KB rows instead of 54 planes; the row blocks of a z-plane are
+// contiguous, and so are the planes.  An array may exceed 2^32 elements, one
+// z-plane may not: addresses are a plane base (64-bit, uniform where z is)
+// plus a 32-bit offset within the plane.
+//   pinplane: offset of interior column (x,y) within its plane (row 0)
+//   plane_ptr: base of z-plane z of a population array
+//   pidx: the element index modulo 2^32, the key of the stochastic rounding
+__device__ __forceinline__ uint32_t pinplane(const DevParams &P, int x, int y) {
+    return uint32_t(x - 1) + P.rowstride * uint32_t(y - 1);
+}
+__device__ __forceinline__ const void *plane_ptr(const DevParams &P, const void *f, int z) {
+    return static_cast<const char *>(f) + size_t(P.planeel) * size_t(z - 1) * 4;
+}
+__device__ __forceinline__ void *plane_ptr(const DevParams &P, void *f, int z) {
+    return static_cast<char *>(f) + size_t(P.planeel) * size_t(z - 1) * 4;
+}
 __device__ __forceinline__ uint32_t pidx(const DevParams &P, int x, int y, int z) {
-    return uint32_t(x - 1) + P.rowstride * (uint32_t(y - 1) + uint32_t(P.IY) * uint32_t(z - 1));
+    return pinplane(P, x, y) + P.planeel * uint32_t(z - 1);
+}
+// the same element index in 64 bits (arrays in the node layout)
+__device__ __forceinline__ size_t pidx64(const DevParams &P, int x, int y, int z) {
+    return size_t(P.planeel) * size_t(z - 1) + pinplane(P, x, y);
 }
 // moment-plane index of interior node (x,y,z)
 __device__ __forceinline__ uint32_t midx(const DevParams &P, int x, int y, int z) {
@@ -155,7 +175,7 @@ __device__ __forceinline__ float A_node(const DevParams &P, float rr, float rb) 
 
 // ---- population storage ------------------------------------------------
 // Direction I of colour c sits at element (c*27 + I)*stride + off of the
-// node's row block (stride = PX, off = pidx).
+// node's z-plane (stride = PX, off = pinplane; f = plane_ptr).
 // FP32: two rows per direction ([colour*27 + i]).
 // FP16 / BF16: one 32-bit word per direction and node: the low half holds
 // t/w_i - 1 with t = f_r + f_b (the offset of the two-colour population from
@@ -219,9 +239,9 @@ struct Store<MCG_STORE_FP32> {
 template <int ST>
 struct Store16 {
     static constexpr int planes = Q;
-    // word idx of the population array as a 32-bit byte offset from its base:
-    // the arrays of 16-bit storage are limited to 4 GiB, so the loads and
-    // stores address base (scalar) + offset (32-bit vector register)
+    // word idx of a z-plane as a 32-bit byte offset from the plane's base
+    // (plane_ptr; a plane stays below 4 GiB), so the loads and stores address
+    // base (scalar) + offset (32-bit vector register)
     __device__ __forceinline__ static const uint32_t *at(const void *f, uint32_t idx) {
         return reinterpret_cast<const uint32_t *>(static_cast<const char *>(f) + (idx << 2));
     }
@@ -346,11 +366,13 @@ template <>
 struct Store<MCG_STORE_BF16> : Store16<MCG_STORE_BF16> {};
 
 // Source of the population that streams into interior node (x,y,z) along
-// direction I: kind 0 = interior node (off = its plane offset), 1 / 2 = the
-// inlet / outlet source buffer (off = element of colour r, colour b at +NX*NZ),
-// 3 = bounce-back at an x/z wall (off = the node itself, direction opp(I)).
+// direction I: kind 0 = interior node (z = its plane, off = its offset within
+// the plane), 1 / 2 = the inlet / outlet source buffer (off = element of colour
+// r, colour b at +NX*NZ), 3 = bounce-back at an x/z wall (the node itself,
+// direction opp(I)).
 struct Src {
     int kind;
+    int z;
     size_t off;
 };
 
@@ -361,7 +383,7 @@ __device__ __forceinline__ Src source_of(const DevParams &P, int x, int y, int z
     int zs = z - kCZ[I];
     // half-way bounce-back: a population arriving from a wall ghost is the
     // node's own post-collision population of the opposite direction
-    if (P.wall && (xs < 1 || xs > P.NX - 2 || zs < 1 || zs > P.NZ - 2)) return Src{3, size_t(pidx(P, x, y, z))};
+    if (P.wall && (xs < 1 || xs > P.NX - 2 || zs < 1 || zs > P.NZ - 2)) return Src{3, z, size_t(pinplane(P, x, y))};
     if constexpr (kCY[I] != 0) {
         if (P.y_mode == MCG_Y_JET) {
             const bool ghost = kCY[I] > 0 ? (ys == 0) : (ys == P.NY - 1);
@@ -369,7 +391,7 @@ __device__ __forceinline__ Src source_of(const DevParams &P, int x, int y, int z
                 // ghost-plane sources are not wrapped (jet/boundary_conditions.cuh:73,222)
                 const size_t nxz = size_t(P.NX) * size_t(P.NZ);
                 const size_t off = size_t(slot_of(I)) * 2 * nxz + size_t(zs) * P.NX + size_t(xs);
-                return Src{kCY[I] > 0 ? 1 : 2, off};
+                return Src{kCY[I] > 0 ? 1 : 2, 0, off};
             }
         } else {
             ys = wrapN(ys, P.NY);
@@ -377,7 +399,7 @@ __device__ __forceinline__ Src source_of(const DevParams &P, int x, int y, int z
     }
     xs = wrapN(xs, P.NX);
     zs = wrapN(zs, P.NZ);
-    return Src{0, size_t(pidx(P, xs, ys, zs))};
+    return Src{0, zs, size_t(pinplane(P, xs, ys))};
 }
 
 // g: f_r + f_b as the moments sum it (Store::load)
@@ -387,9 +409,9 @@ __device__ __forceinline__ void gather(const DevParams &P, const void *__restric
                                        float &g) {
     const Src s = source_of<I>(P, x, y, z);
     if (s.kind == 0) {
-        Store<ST>::template load<I>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb, g);
+        Store<ST>::template load<I>(plane_ptr(P, f, s.z), uint32_t(P.PX), uint32_t(s.off), fr, fb, g);
     } else if (s.kind == 3) {
-        Store<ST>::template load<opp(I)>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb, g);
+        Store<ST>::template load<opp(I)>(plane_ptr(P, f, s.z), uint32_t(P.PX), uint32_t(s.off), fr, fb, g);
     } else {
         const float *b = s.kind == 1 ? bin : bout;
         fr = b[s.off];
@@ -409,9 +431,9 @@ __device__ __forceinline__ void scatter(const DevParams &P, void *f, float *bin,
                                         float fr, float fb) {
     const Src s = source_of<I>(P, x, y, z);
     if (s.kind == 0) {
-        Store<ST>::template store<I>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb);
+        Store<ST>::template store<I>(plane_ptr(P, f, s.z), uint32_t(P.PX), uint32_t(s.off), fr, fb);
     } else if (s.kind == 3) {
-        Store<ST>::template store<opp(I)>(f, uint32_t(P.PX), uint32_t(s.off), fr, fb);
+        Store<ST>::template store<opp(I)>(plane_ptr(P, f, s.z), uint32_t(P.PX), uint32_t(s.off), fr, fb);
     } else {
         float *b = s.kind == 1 ? bin : bout;
         b[s.off] = fr;
@@ -565,14 +587,17 @@ __device__ __forceinline__ uint32_t sr_node_key(uint32_t p, uint32_t step_key) {
 // halves of both words, G = 0x9E3779B9 (a Weyl sequence: distinct high bits
 // for the 14 groups).  Every stored half is unbiased whatever the sharing;
 // sharing only correlates the noise within a node.
+// f: the node's z-plane (plane_ptr), p: its offset there, key: its element
+// index (pidx), which keys the stochastic rounding
 template <int ST, bool BULK>
-__device__ __forceinline__ void collide_store_as(const NodeColl &n, void *f, uint32_t PX, uint32_t p, uint32_t step_key) {
+__device__ __forceinline__ void collide_store_as(const NodeColl &n, void *f, uint32_t PX, uint32_t p, uint32_t key,
+                                                 uint32_t step_key) {
     if constexpr (ST == MCG_STORE_FP32) {
         collide_node<BULK>(n, [&](auto Ic, float a, float b) {
             Store<ST>::template store<decltype(Ic)::value>(f, PX, p, a, b);
         });
     } else {
-        const uint32_t h = sr_node_key(p, step_key);
+        const uint32_t h = sr_node_key(key, step_key);
         {
             float E = fma_(n.q2, n.tr, n.e0);
             if constexpr (!BULK) E = fma_(n.sB, 1.25f, E);
@@ -595,15 +620,16 @@ __device__ __forceinline__ void collide_store_as(const NodeColl &n, void *f, uin
 // faster, 192x384x192 rising bubble 4.5%).  The 16-bit steps keep the one
 // general form: the second copy of the collision ran them 1.5-4% slower.
 template <int ST>
-__device__ __forceinline__ void collide_store(const NodeColl &n, void *f, uint32_t PX, uint32_t p, uint32_t step_key) {
+__device__ __forceinline__ void collide_store(const NodeColl &n, void *f, uint32_t PX, uint32_t p, uint32_t key,
+                                              uint32_t step_key) {
     if constexpr (ST == MCG_STORE_FP32) {
         const bool bulk = n.s2 == 0.0f && n.sB == 0.0f && n.rc == 0.0f;
         if (__all(bulk)) {
-            collide_store_as<ST, true>(n, f, PX, p, step_key);
+            collide_store_as<ST, true>(n, f, PX, p, key, step_key);
             return;
         }
     }
-    collide_store_as<ST, false>(n, f, PX, p, step_key);
+    collide_store_as<ST, false>(n, f, PX, p, key, step_key);
 }
 
 // SplitMix64 (Steele, Lea and Flood 2014, the finaliser constants of Vigna's

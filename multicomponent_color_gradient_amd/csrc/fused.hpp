@@ -39,16 +39,18 @@ __device__ __forceinline__ DevParams fresh_strides(const DevParams &P) {
         Q2.f = decltype(Q2.f)(v_);                      \
     }
     MCG_LAUNDER(NX) MCG_LAUNDER(NY) MCG_LAUNDER(NZ) MCG_LAUNDER(PX) MCG_LAUNDER(IY) MCG_LAUNDER(IZ)
-    MCG_LAUNDER(plane) MCG_LAUNDER(rowstride) MCG_LAUNDER(y_mode) MCG_LAUNDER(wrap_gradient)
+    MCG_LAUNDER(plane) MCG_LAUNDER(rowstride) MCG_LAUNDER(planeel) MCG_LAUNDER(y_mode) MCG_LAUNDER(wrap_gradient)
 #undef MCG_LAUNDER
     return Q2;
 }
 
-// Pull source of an interior node with the periodic wraps folded into
-// per-lane offset corrections (same addresses as source_of()).
+// Pull source of an interior node with the periodic x/y wraps folded into
+// per-lane offset corrections within the z-plane and the z wrap into the
+// choice of plane (same addresses as source_of()).
 struct FastSrc {
-    uint32_t p;
-    uint32_t wxp, wxm, wyp, wym, wzp, wzm;  // modulo-2^32 offset corrections
+    uint32_t p;                   // offset of the node's column within a plane (pinplane)
+    uint32_t wxp, wxm, wyp, wym;  // modulo-2^32 offset corrections
+    int z, zp, zm;                // the node's plane and the planes it pulls c_z = +1 / -1 populations from
     int o;
     bool yin, yout;
     bool bxp, bxm, bzp, bzm;  // x/z walls: sources across x=0, x=NX-1, z=0, z=NZ-1 bounce back
@@ -56,13 +58,15 @@ struct FastSrc {
 
 __device__ __forceinline__ FastSrc make_src(const DevParams &P, int x, int y, int z) {
     FastSrc s;
-    s.p = pidx(P, x, y, z);
-    const uint32_t dy = P.rowstride, dz = P.rowstride * uint32_t(P.IY);
+    s.p = pinplane(P, x, y);
+    const uint32_t dy = P.rowstride;
     const bool wall = P.wall != 0;
     s.wxp = (!wall && x == 1) ? uint32_t(P.NX - 2) : 0u;
     s.wxm = (!wall && x == P.NX - 2) ? 0u - uint32_t(P.NX - 2) : 0u;
-    s.wzp = (!wall && z == 1) ? uint32_t(P.NZ - 2) * dz : 0u;
-    s.wzm = (!wall && z == P.NZ - 2) ? 0u - uint32_t(P.NZ - 2) * dz : 0u;
+    // with walls, a source across z = 0 or NZ-1 bounces back and its plane is unused
+    s.z = z;
+    s.zp = (!wall && z == 1) ? P.NZ - 2 : z - 1;
+    s.zm = (!wall && z == P.NZ - 2) ? 1 : z + 1;
     s.bxp = wall && x == 1;
     s.bxm = wall && x == P.NX - 2;
     s.bzp = wall && z == 1;
@@ -109,19 +113,18 @@ __device__ __forceinline__ void fast_load(const DevParams &P, const void *__rest
         if constexpr (cz > 0) bb = bb || s.bzp;
         if constexpr (cz < 0) bb = bb || s.bzm;
         if (bb) {  // half-way bounce-back
-            Store<ST>::template load<opp(I)>(f, uint32_t(P.PX), s.p, fr, fb, g);
+            Store<ST>::template load<opp(I)>(plane_ptr(P, f, s.z), uint32_t(P.PX), s.p, fr, fb, g);
             return;
         }
     }
     // unsigned arithmetic: intermediate values may wrap, the final offset is in range
-    uint32_t off = s.p - uint32_t(cx) - uint32_t(cy) * P.rowstride - uint32_t(cz) * (P.rowstride * uint32_t(P.IY));
+    uint32_t off = s.p - uint32_t(cx) - uint32_t(cy) * P.rowstride;
     if constexpr (cx > 0) off += s.wxp;
     if constexpr (cx < 0) off += s.wxm;
     if constexpr (cy > 0) off += s.wyp;
     if constexpr (cy < 0) off += s.wym;
-    if constexpr (cz > 0) off += s.wzp;
-    if constexpr (cz < 0) off += s.wzm;
-    Store<ST>::template load<I>(f, uint32_t(P.PX), off, fr, fb, g);
+    const int zs = cz > 0 ? s.zp : (cz < 0 ? s.zm : s.z);
+    Store<ST>::template load<I>(plane_ptr(P, f, zs), uint32_t(P.PX), off, fr, fb, g);
 }
 
 template <int ST>
@@ -176,13 +179,12 @@ __device__ __forceinline__ void raw_issue(const DevParams &P0, const DevBufs &B,
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
         constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
-        uint32_t off = s.p - uint32_t(cx) - uint32_t(cy) * P.rowstride - uint32_t(cz) * (P.rowstride * uint32_t(P.IY));
+        uint32_t off = s.p - uint32_t(cx) - uint32_t(cy) * P.rowstride;
         if constexpr (cx > 0) off += s.wxp;
         if constexpr (cx < 0) off += s.wxm;
         if constexpr (cy > 0) off += s.wyp;
         if constexpr (cy < 0) off += s.wym;
-        if constexpr (cz > 0) off += s.wzp;
-        if constexpr (cz < 0) off += s.wzm;
+        int zs = cz > 0 ? s.zp : (cz < 0 ? s.zm : s.z);
         if constexpr (cx != 0 || cz != 0) {
             // half-way bounce-back at an x/z wall: the node's own word of the
             // opposite direction (row opp(I), one PX away), as fast_load reads it
@@ -191,9 +193,12 @@ __device__ __forceinline__ void raw_issue(const DevParams &P0, const DevBufs &B,
             if constexpr (cx < 0) bb = bb || s.bxm;
             if constexpr (cz > 0) bb = bb || s.bzp;
             if constexpr (cz < 0) bb = bb || s.bzm;
-            if (bb) off = s.p + uint32_t(opp(I) - I) * uint32_t(P.PX);
+            if (bb) {
+                off = s.p + uint32_t(opp(I) - I) * uint32_t(P.PX);
+                zs = s.z;
+            }
         }
-        Store<ST>::template load_raw<I>(B.f_src, uint32_t(P.PX), off, r.w + I * Store<ST>::kWords);
+        Store<ST>::template load_raw<I>(plane_ptr(P, B.f_src, zs), uint32_t(P.PX), off, r.w + I * Store<ST>::kWords);
     });
 }
 
@@ -349,6 +354,8 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         // recomputed per plane instead of becoming loop-carried pointers
         int z = zi;
         asm volatile("" : "+s"(z));
+        // marked uniform again, so that the plane bases stay scalar
+        z = __builtin_amdgcn_readfirstlane(z);
         const int zl = z + 2;
         const bool lead = zl <= ze + 1;
         int zln = zl;
@@ -368,8 +375,8 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
             };
             const NodeColl n = node_setup(P, m, y, phi);
             const DevParams PL = fresh_strides(P);
-            const uint32_t p = pidx(PL, x, y, z);
-            collide_store<ST>(n, B.f_dst, uint32_t(PL.PX), p, sr_step_key(B.step));
+            collide_store<ST>(n, plane_ptr(PL, B.f_dst, z), uint32_t(PL.PX), pinplane(PL, x, y), pidx(PL, x, y, z),
+                              sr_step_key(B.step));
             if constexpr (MOM) {
                 const uint32_t q = midx(PL, x, y, z);
                 const size_t pl = PL.plane;

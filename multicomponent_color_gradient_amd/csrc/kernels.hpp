@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_collide(DevParams P, DevBufs B) {
         return ghost_psi(P, B, xn, yn, zn);
     };
     const NodeColl n = node_setup(P, m, y, phi);
-    collide_store<ST>(n, B.f_dst, uint32_t(P.PX), p, sr_step_key(B.step));
+    collide_store<ST>(n, plane_ptr(P, B.f_dst, z), uint32_t(P.PX), pinplane(P, x, y), p, sr_step_key(B.step));
     if (P.y_mode == MCG_Y_JET) {
         if (y == 1 && is_jet(P, x, z)) inlet_emit(P, B.in_dst, m, x, z, B.step, B.inlet_du);
         if (y == P.NY - 2) outlet_emit(P, B.gB_src, B.gB_dst, B.out_dst, m, x, z);

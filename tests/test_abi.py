@@ -75,6 +75,26 @@ def test_case_validation_accepts_baseline_grids():
         assert lib.mcg_case_validate(ctypes.byref(c)) == 0
 
 
+def test_case_validation_grid_limits():
+    """Population arrays may exceed 2^32 elements (addresses are a z-plane base
+    plus a 32-bit offset within the plane); one z-plane must stay below 4 GiB
+    and the padded interior below 2^32 nodes."""
+    lib = m.load_library()
+    # 512^3: 7.2e9 FP32 population elements per buffer, planes of 56 MB
+    c = m.jet_case(512, 512, 512, 16.0)
+    assert lib.mcg_case_validate(ctypes.byref(c)) == 0
+    # 130 x 130 x 5002, the z-translation test's box: 4.4e9 elements
+    c = m.jet_case(130, 130, 5002, 8.0)
+    assert lib.mcg_case_validate(ctypes.byref(c)) == 0
+    # a plane of 2048 x 10000 nodes: 4.4 GB of FP32 populations
+    c = m.jet_case(2048, 10000, 8, 8.0)
+    assert lib.mcg_case_validate(ctypes.byref(c)) == _abi.MCG_E_INVALID
+    assert b"z-plane" in lib.mcg_last_error()
+    # 2048 x 2048 x 1100 padded interior nodes exceed 2^32
+    c = m.jet_case(2048, 2048, 1100, 8.0)
+    assert lib.mcg_case_validate(ctypes.byref(c)) == _abi.MCG_E_INVALID
+
+
 def test_errors_without_gpu_are_status_codes():
     """Null handles and pointers are rejected with a status, never a crash."""
     lib = m.load_library()

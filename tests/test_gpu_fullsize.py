@@ -9,7 +9,11 @@ and the 192x384x192 periodic bubble.
     rounding (1e-6 relative after 50 steps) and keeps zero net momentum;
   * FP16-offset storage stays within 1e-3 U of FP32 in u (BF16 within 2e-2 U)
     and within 1e-5 in total mass after 100 jet steps;
-  * the checkpoint round trip is bitwise at full size.
+  * the checkpoint round trip is bitwise at full size;
+  * population arrays beyond 2^32 elements (FP32) and 4 GiB (16-bit): a
+    periodic bubble in a tall box gives bitwise the same fields whether it
+    sits in the first planes or in planes past the 32-bit offset range, and
+    the fused and two-pass kernels agree bitwise on an 8.8 GB 16-bit buffer.
 """
 import numpy as np
 import pytest
@@ -116,3 +120,51 @@ def test_launch_tuning_keeps_results_bitwise():
     for fa, ca, _ in out[1:]:
         assert all(np.array_equal(out[0][0][k], fa[k]) for k in fa)
         assert np.array_equal(out[0][1], ca)
+
+
+def tall_bubble(z0, storage="fp32"):
+    """130 x 130 x 5002 periodic box, tanh bubble R=10 centred at z0: FP32
+    populations of 4.4e9 elements (18 GB) per buffer, 8.8 GB with 16-bit storage."""
+    c = m.periodic_bubble_case(130, radius=10.0, tau=0.8, A=1e-3, beta=0.7)
+    c.NZ = 5002
+    c.bubble_z0 = float(z0)
+    return c
+
+
+def test_planes_past_32bit_offsets_translate_bitwise():
+    """z-translation invariance of the periodic system: the same bubble near
+    z = 64 and near z = 4900 (planes whose elements lie beyond 2^32 from the
+    array's start) gives bitwise the same shifted fields after 20 steps."""
+    shift = 4900 - 64
+    win = slice(64 - 40, 64 + 41)
+    out = []
+    for z0 in (64.5, 64.5 + shift):
+        print(f"past 32-bit offsets: 130x130x5002 periodic bubble at z0={z0}, 20 steps", flush=True)
+        s = m.Solver(tall_bubble(z0))
+        assert s.device_bytes() > 2 * 4 * (1 << 32)
+        s.init_bubble_tanh()
+        s.step(20)
+        f = s.fields()
+        zs = slice(win.start + (0 if z0 < 1000 else shift), win.stop + (0 if z0 < 1000 else shift))
+        out.append({k: np.array(f[k][zs]) for k in ("rhor", "rhob", "ux", "uy", "uz")})
+        del f
+        s.close()
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+    # the window holds the bubble and its perturbation
+    assert np.max(out[0]["rhob"]) > 0.9 and np.max(np.abs(out[0]["uz"])) > 0.0
+
+
+def test_16bit_buffer_past_4gib_fused_equals_twopass():
+    print("past 4 GiB: 130x130x5002 FP16 bubble at z0=4900.5, fused vs twopass, 10 steps", flush=True)
+    res = []
+    for kernel in ("fused", "twopass"):
+        s = m.Solver(tall_bubble(4900.5), storage="fp16", kernel=kernel)
+        s.init_bubble_tanh()
+        s.step(10)
+        f = s.fields()
+        res.append((np.array(f["rhob"][4850:4950]), np.array(f["uz"][4850:4950])))
+        del f
+        s.close()
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
