@@ -1,23 +1,26 @@
 // fused.hpp -- the one-launch step kernel.
 //
 // Work decomposition: a workgroup owns a tile of 64 x-columns by TY y-rows
-// (one wavefront per row) and marches along z through a segment of planes.
-// For each plane z it
-//   A. gathers the 54 populations of its nodes on the lead plane z+1 (pull
-//      streaming), reduces them to moments kept in registers, and writes
-//      psi = (rho_r - rho_b)/(rho_r + rho_b) of the lead plane into a 4-deep
-//      LDS ring; the one-node halo ring (rows y0-1, y0+TY, columns x0-1,
-//      x0+64) is gathered too, densities only;
-//   B. after one barrier, collides plane z from the moments held in registers
-//      and the psi of planes z-1, z, z+1 read from LDS, and stores the 54
-//      post-collision populations at the node (aligned, coalesced rows).
-// HBM traffic per interior node is one read and one write of 54 values; the
-// halo re-reads hit L2 because neighbouring tiles of a segment run on the
-// same XCD (workgroup ids are remapped so that each XCD owns a contiguous
-// block of tiles).  Radial statistics, when due, accumulate in an LDS
-// histogram (rows of the tile x radial bins) flushed once per workgroup.
-// The arithmetic per node is the same code as the two-pass path, so both
-// paths produce identical populations.
+// (one wavefront per row, plus three halo waves) and marches along z through
+// a segment of planes.  Iteration z
+//   A. issues the loads of the lead plane z+2 (pull streaming: 27 or 54 words
+//      per node) before anything else, so they are in flight during B;
+//   B. collides plane z from its moments (an LDS ring, or registers on
+//      statistics steps) and the psi of planes z-1, z, z+1 (a 4-plane LDS
+//      ring), and stores the post-collision populations at the node
+//      (aligned, coalesced, non-temporal rows);
+//   C. reduces the lead plane's words to moments and psi, writes them to the
+//      rings, and passes the one barrier of the iteration.  The halo waves
+//      gather the one-node ring around the tile (rows y0-1, y0+TY, columns
+//      x0-1, x0+64), psi only.
+// HBM traffic per interior node is one read and one write of its
+// populations; the halo and edge-line re-reads mostly hit L2 because
+// neighbouring tiles of a segment run on the same XCD (workgroup ids are
+// remapped so that each XCD owns a contiguous block of tiles).  Radial
+// statistics, when due, accumulate in an LDS histogram (rows of the tile x
+// radial bins) flushed once per workgroup.  The arithmetic per node is the
+// same code as the two-pass path, so both paths produce identical
+// populations.
 #pragma once
 
 #include <algorithm>
@@ -290,6 +293,7 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     using Shape = FusedShape<ST, STATS>;
     constexpr int TY = Shape::TY;
     constexpr bool MLDS = Shape::kMomLds;
+    constexpr bool kEarlySerial = MLDS && ST == MCG_STORE_FP32;
     constexpr int H = TY + 2;
     constexpr int W = 66;
     constexpr int NT = 64 * Shape::kWaves;
@@ -327,6 +331,35 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         lr = tx >> 1;
         lc = (tx & 1) ? W - 1 : 0;
     }
+    // the tile row whose moments this lane's gather yields (-1: psi only)
+    int mrow = core ? row : -1;
+    if constexpr (MLDS) {
+        // Jet: the inlet row (y = 1) and the outlet row (y = NY-2) gather
+        // serially (their sources include the boundary buffers), and with
+        // 16-bit storage after the collision, while the workgroup waits at
+        // the barrier: the tiles holding them ran 1.8 times as long as the
+        // others.  The row-halo wave beside such a row has almost nothing to
+        // gather (the ghost inlet row, or a row past the outlet), so the two
+        // waves trade positions: the halo wave gathers the boundary row and
+        // hands its moments over through the ring, and the core wave only
+        // collides.
+        if (P.y_mode == MCG_Y_JET) {
+            const int rin = 1 - y0, rout = P.NY - 2 - y0;
+            if (hrole == 0 && rin >= 0 && rin < TY) {
+                lr = rin + 1;
+                mrow = rin;
+            } else if (hrole == 1 && rout >= 0 && rout < TY && rout != rin) {
+                lr = rout + 1;
+                mrow = rout;
+            } else if (core && row == rin) {
+                lr = 0;
+                mrow = -1;
+            } else if (core && row == rout) {
+                lr = H - 1;
+                mrow = -1;
+            }
+        }
+    }
     const int gx = x0 - 1 + lc, gy = y0 - 1 + lr;
 
     if constexpr (STATS) {
@@ -362,6 +395,26 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         if (P.wrap_gradient && !P.wall) zln = wrapN(zl, P.NZ);
         const bool split = lead && split_xy && zln >= 1 && zln <= P.NZ - 2;
         if (split) raw_issue<ST>(P, B, gxn, gyn, zln, raw);
+        if constexpr (kEarlySerial) {
+            // FP32: a lane that cannot split its gather (positions off the
+            // interior, the jet's boundary rows) gathers before the collision,
+            // so its wave waits on the loads while the other waves of its SIMD
+            // collide (at 256x512x256 the jet step ran 3% faster, the rising
+            // bubble 4%).  With 16-bit storage the same order spills 12 to 40
+            // more VGPRs.  The ring slots of plane zl are read by no one
+            // before the barrier.
+            if (lead && active && !split) {
+                bool has_m;
+                const float ph = gather_pos<ST>(P, B, gx, gy, zl, nx2, has_m);
+                if (mrow >= 0) {
+                    float *ms = mring + (size_t((zl % 3) * TY + mrow) * NMOM) * 64 + tx;
+                    const float v[NMOM] = {nx2.rr, nx2.rb, nx2.ux, nx2.uy, nx2.uz, nx2.pxx,
+                                           nx2.pxy, nx2.pyy, nx2.pyz, nx2.pzz, nx2.pxz};
+                    for (int k = 0; k < NMOM; ++k) ms[k * 64] = v[k];
+                }
+                phs[zl & 3][lr][lc] = ph;
+            }
+        }
         if (node && z >= zb) {
             if constexpr (MLDS) {
                 const float *ms = mslot(z);
@@ -398,20 +451,20 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
                 }
             }
         }
-        if (lead && active) {
-            float ph;
+        if (lead && active && (!kEarlySerial || split)) {
+            float ph = 0.0f;
             if (split && !core) {
                 ph = raw_psi<ST>(raw);  // a halo wave: only psi is used
             } else if (split) {
                 nx2 = raw_reduce<ST>(raw, P.ghalf);
                 ph = psi(nx2.rr, nx2.rb);
-            } else {
+            } else if constexpr (!kEarlySerial) {
                 bool has_m;
                 ph = gather_pos<ST>(P, B, gx, gy, zl, nx2, has_m);
             }
             if constexpr (MLDS) {
-                if (core) {
-                    float *ms = mslot(zl);
+                if (mrow >= 0) {
+                    float *ms = mring + (size_t((zl % 3) * TY + mrow) * NMOM) * 64 + tx;
                     const float v[NMOM] = {nx2.rr, nx2.rb, nx2.ux, nx2.uy, nx2.uz, nx2.pxx,
                                            nx2.pxy, nx2.pyy, nx2.pyz, nx2.pzz, nx2.pxz};
                     for (int k = 0; k < NMOM; ++k) ms[k * 64] = v[k];
