@@ -143,16 +143,20 @@ int mcg_set_kernel(mcg_solver *s, int kernel);
 
 /* Launch shape of the fused step (an extension; the reference launches fixed
  * block shapes, utilities/cudaConfig.cuh:22-31).  The step kernel cuts z into
- * segments.  segments = 0 (the default): the solver's first plain steps (no
- * statistics) try the candidate segment counts in turn, each timed with HIP
- * events and waited for, and keep the fastest of two passes.  segments > 0:
- * the candidate nearest to that many segments, fixed, with no timing.  The
- * results do not depend on the segmentation. */
+ * segments of equal length (uniform shapes) or, per XCD, into z-bands of
+ * decreasing thickness (banded schedules).  segments = 0 (the default): the
+ * solver's first plain steps (no statistics) try the candidate shapes in turn,
+ * each timed with HIP events and waited for, and keep the fastest of two
+ * passes.  segments > 0: the uniform shape nearest to that many segments,
+ * fixed, with no timing.  segments = -k: the k-th banded schedule (1 = the
+ * shortest estimated span), fixed; MCG_E_INVALID if there is none.  The
+ * results do not depend on the launch shape. */
 int mcg_set_launch_segments(mcg_solver *s, int segments);
 
-/* The current launch shape: z segments, workgroups per step, and whether the
- * timing has chosen it.  Any pointer may be NULL. */
-int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned);
+/* The current launch shape: z segments (or the z-bands of a banded
+ * schedule), workgroups per step, whether the timing has chosen it, and
+ * whether it is banded.  Any pointer may be NULL. */
+int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned, int32_t *banded);
 
 /* ---- case API: initial conditions ------------------------------------- */
 

@@ -113,6 +113,7 @@ struct mcg_solver {
     // candidate z segmentations timed on the first plain steps (two passes)
     FusedGrid grid{};
     std::vector<FusedGrid> tune_cand;
+    FusedUnit *units = nullptr;  // unit tables of the banded candidates (one allocation)
     std::vector<float> tune_ms;
     int tune_next = 0;
     bool tune_on = true, tuned = false;
@@ -602,6 +603,32 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
     s->grid = fused_grid(s->P, s->num_cu);
     s->tune_cand = fused_candidates(s->P);
     {
+        // the banded schedules join the timed candidates
+        const std::vector<BandedSchedule> banded = banded_candidates(s->P, s->num_cu);
+        size_t total = 0;
+        for (const BandedSchedule &b : banded) total += b.units.size();
+        if (total) {
+            std::vector<FusedUnit> all;
+            all.reserve(total);
+            for (const BandedSchedule &b : banded) all.insert(all.end(), b.units.begin(), b.units.end());
+            if (hipMalloc((void **)&s->units, total * sizeof(FusedUnit)) != hipSuccess ||
+                hipMemcpy(s->units, all.data(), total * sizeof(FusedUnit), hipMemcpyHostToDevice) != hipSuccess) {
+                if (s->units) (void)hipFree(s->units);
+                delete s;
+                return fail(MCG_E_NOMEM, "launch schedule tables: device allocation or upload failed");
+            }
+            size_t off = 0;
+            for (const BandedSchedule &b : banded) {
+                FusedGrid G = fused_grid_for(s->P, b.bands, fused_plain_ty(s->P.storage));
+                G.nzs = b.bands;
+                G.nwg = int(b.units.size());
+                G.units = s->units + off;
+                off += b.units.size();
+                s->tune_cand.push_back(G);
+            }
+        }
+    }
+    {
         // histograms larger than the default 64 KiB need the kernel's dynamic-LDS limit raised
         int lds = 0;
         const size_t need = fused_stats_lds(s->P) + fused_static_lds();
@@ -664,6 +691,7 @@ int mcg_destroy(mcg_solver *s) {
         if (p) (void)hipFree(p);
     for (hipEvent_t e : s->tune_ev)
         if (e) (void)hipEventDestroy(e);
+    if (s->units) (void)hipFree(s->units);
     s->release_staging();
     s->adj_release();
     s->uy_release();
@@ -681,27 +709,42 @@ int mcg_set_kernel(mcg_solver *s, int kernel) {
 
 int mcg_set_launch_segments(mcg_solver *s, int segments) {
     if (int rc = check_solver(s)) return rc;
-    if (segments < 0) return fail(MCG_E_INVALID, "segments must be >= 0");
+    if (segments < 0) {
+        int k = 0;
+        for (const FusedGrid &G : s->tune_cand) {
+            if (G.units && ++k == -segments) {
+                s->tuned = false;
+                s->tune_next = 0;
+                s->tune_ms.clear();
+                s->tune_on = false;
+                s->grid = G;
+                return MCG_OK;
+            }
+        }
+        return fail(MCG_E_INVALID, "no banded launch schedule " + std::to_string(-segments) + " (" + std::to_string(k) +
+                                       " for this grid)");
+    }
     s->tuned = false;
     s->tune_next = 0;
     s->tune_ms.clear();
     s->tune_on = segments == 0;
     s->grid = fused_grid(s->P, s->num_cu);
     if (segments > 0) {
-        // the candidate with the segment count nearest to the one asked for
+        // the uniform candidate with the segment count nearest to the one asked for
         const FusedGrid *best = &s->tune_cand.front();
         for (const FusedGrid &G : s->tune_cand)
-            if (std::abs(G.nzs - segments) < std::abs(best->nzs - segments)) best = &G;
+            if (!G.units && std::abs(G.nzs - segments) < std::abs(best->nzs - segments)) best = &G;
         s->grid = *best;
     }
     return MCG_OK;
 }
 
-int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned) {
+int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned, int32_t *banded) {
     if (int rc = check_solver(s)) return rc;
     if (segments) *segments = s->grid.nzs;
     if (workgroups) *workgroups = s->grid.nwg;
     if (tuned) *tuned = s->tuned ? 1 : 0;
+    if (banded) *banded = s->grid.units ? 1 : 0;
     return MCG_OK;
 }
 

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "schedule.hpp"
 
 namespace mcg {
 
@@ -287,9 +288,14 @@ __device__ __forceinline__ float raw_psi(const RawGather<ST> &r) {
     return psi(sr, sb);
 }
 
+// Launch shape of the step kernel.  units == nullptr: the uniform shape, nzs
+// segments of zlen planes per tile, workgroup b on tile xcd_tile(b).
+// Otherwise a banded schedule: workgroup b runs units[b] (nzs then counts its
+// z-bands, and instances re-tiled to other rows use nzs uniform segments).
 struct FusedGrid {
     int ty;  // core rows per tile of the instance the grid was made for
     int nxt, nyb, nzs, zlen, nwg;
+    const FusedUnit *units = nullptr;
 };
 
 // Tile shape of a k_fused instance: TY core waves (one y-row of 64 x-columns
@@ -357,16 +363,23 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
     __shared__ float mring[MLDS ? 3 * TY * NMOM * 64 : 1];
     extern __shared__ unsigned long long hist[];  // STATS: [TY][NR][8], 7 fixed-point sums and the count
 
-    const int t = xcd_tile(int(blockIdx.x), G.nwg);
-    const int xt = t % G.nxt;
-    const int r1 = t / G.nxt;
-    const int yb = r1 % G.nyb;
-    const int zs = r1 / G.nyb;
+    int tile, zb, ze;
+    if (G.units) {
+        const FusedUnit u = G.units[blockIdx.x];
+        tile = int(u.tile);
+        zb = int(u.zb);
+        ze = int(u.ze);
+        if (ze < zb) return;  // padding: the whole workgroup leaves before any barrier
+    } else {
+        const int t = xcd_tile(int(blockIdx.x), G.nwg);
+        tile = t % (G.nxt * G.nyb);
+        zb = 1 + (t / (G.nxt * G.nyb)) * G.zlen;
+        ze = min(P.NZ - 2, zb + G.zlen - 1);
+    }
+    const int xt = tile % G.nxt, yb = tile / G.nxt;
     const int tx = int(threadIdx.x), wv = int(threadIdx.y);
     const int lid = wv * 64 + tx;
     const int x0 = 1 + xt * 64, y0 = 1 + yb * TY;
-    const int zb = 1 + zs * G.zlen;
-    const int ze = min(P.NZ - 2, zb + G.zlen - 1);
     const bool core = wv < TY;
     const int row = core ? wv : 0;
     const int hrole = core ? -1 : wv - TY;   // halo role 0..2
@@ -586,6 +599,15 @@ inline std::vector<FusedGrid> fused_candidates(const DevParams &P) {
         if (all.empty() || all.back().nzs != G.nzs) all.push_back(G);
     }
     return all;
+}
+
+// The banded candidates of a launch: the kBandedCandidates band rules of
+// shortest estimated span (banded_rules) for the plain instance's tiles.
+inline std::vector<BandedSchedule> banded_candidates(const DevParams &P, int num_cu) {
+    const int ty = fused_plain_ty(P.storage);
+    const int ntiles = (P.PX / 64) * ((P.IY + ty - 1) / ty);
+    const double fill = P.storage == MCG_STORE_FP32 ? 1.5 : 1.9;
+    return banded_rules(ntiles, P.IZ, std::max(1, num_cu / kXcds), fill);
 }
 
 // The segmentation used before (or without) the timing: among the counts that

@@ -66,15 +66,19 @@ class Solver:
 
     # ---- case API (jet/jetIn.cuh, bubble/bubbleIn.cuh)
     def set_launch_segments(self, segments: int = 0) -> None:
-        """0: time the candidate z segmentations of the step kernel on the first
-        plain steps and keep the fastest (default); n > 0: about n segments, fixed."""
+        """0: time the candidate launch shapes of the step kernel on the first
+        plain steps and keep the fastest (default); n > 0: about n uniform z
+        segments, fixed; -k: the k-th banded schedule, fixed."""
         check(self.lib.mcg_set_launch_segments(self._h, int(segments)))
 
     def launch_info(self) -> dict:
-        """The step kernel's launch shape: z segments, workgroups per step, tuned or not."""
-        seg, wg, tuned = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        check(self.lib.mcg_launch_info(self._h, ctypes.byref(seg), ctypes.byref(wg), ctypes.byref(tuned)))
-        return {"segments": seg.value, "workgroups": wg.value, "tuned": bool(tuned.value)}
+        """The step kernel's launch shape: z segments (or z-bands of a banded
+        schedule), workgroups per step, tuned or not, banded or uniform."""
+        seg, wg, tuned, banded = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(self.lib.mcg_launch_info(self._h, ctypes.byref(seg), ctypes.byref(wg), ctypes.byref(tuned),
+                                       ctypes.byref(banded)))
+        return {"segments": seg.value, "workgroups": wg.value, "tuned": bool(tuned.value),
+                "schedule": "banded" if banded.value else "uniform"}
 
     def init_jet(self) -> None:
         check(self.lib.mcg_init_jet(self._h))
