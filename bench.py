@@ -97,7 +97,7 @@ def copy_probe(nbytes: int, reps: int = 3) -> None:
 
 def settle_launch(s, limit: int = 200) -> int:
     """Untimed steps until the solver's launch-shape timing (about 2 steps per
-    candidate segmentation, on steps without statistics) has chosen; returns them."""
+    candidate launch shape, on steps without statistics) has chosen; returns them."""
     n = 0
     while not s.launch_info()["tuned"] and n < limit:
         s.step(4)

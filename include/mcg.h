@@ -155,7 +155,9 @@ int mcg_set_launch_segments(mcg_solver *s, int segments);
 
 /* The current launch shape: z segments (or the z-bands of a banded
  * schedule), workgroups per step, whether the timing has chosen it, and
- * whether it is banded.  Any pointer may be NULL. */
+ * whether it is banded (0: uniform; 1, 2, 3: banded with the tile-planes
+ * dealt to the XCDs in contiguous shares, in chunk pairs k and k+8, or in
+ * chunk pairs k and 15-k).  Any pointer may be NULL. */
 int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned, int32_t *banded);
 
 /* ---- case API: initial conditions ------------------------------------- */

@@ -623,6 +623,7 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
                 G.nzs = b.bands;
                 G.nwg = int(b.units.size());
                 G.units = s->units + off;
+                G.deal = int(b.deal);
                 off += b.units.size();
                 s->tune_cand.push_back(G);
             }
@@ -744,7 +745,7 @@ int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups,
     if (segments) *segments = s->grid.nzs;
     if (workgroups) *workgroups = s->grid.nwg;
     if (tuned) *tuned = s->tuned ? 1 : 0;
-    if (banded) *banded = s->grid.units ? 1 : 0;
+    if (banded) *banded = s->grid.units ? 1 + s->grid.deal : 0;
     return MCG_OK;
 }
 

@@ -296,6 +296,7 @@ struct FusedGrid {
     int ty;  // core rows per tile of the instance the grid was made for
     int nxt, nyb, nzs, zlen, nwg;
     const FusedUnit *units = nullptr;
+    int deal = 0;  // banded: how the tile-planes were dealt to the XCDs (BandDeal)
 };
 
 // Tile shape of a k_fused instance: TY core waves (one y-row of 64 x-columns

@@ -78,7 +78,7 @@ class Solver:
         check(self.lib.mcg_launch_info(self._h, ctypes.byref(seg), ctypes.byref(wg), ctypes.byref(tuned),
                                        ctypes.byref(banded)))
         return {"segments": seg.value, "workgroups": wg.value, "tuned": bool(tuned.value),
-                "schedule": "banded" if banded.value else "uniform"}
+                "schedule": ("uniform", "banded", "banded-paired", "banded-mirrored")[banded.value]}
 
     def init_jet(self) -> None:
         check(self.lib.mcg_init_jet(self._h))

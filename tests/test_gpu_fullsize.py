@@ -101,24 +101,27 @@ def test_checkpoint_roundtrip_at_full_size():
 
 def test_launch_tuning_keeps_results_bitwise():
     """The step kernel's launch shape is timed on the first plain steps and the
-    fastest kept; that choice changes no result: after 60 steps (16-bit
+    fastest kept; that choice changes no result: after 100 steps (16-bit
     storage, statistics every 10 steps) the fields and the radial sums equal
-    those of runs with 1 and 7 fixed uniform segments and with the first two
-    banded schedules bit for bit."""
+    those of runs with 1 and 7 fixed uniform segments and with banded
+    schedules (two band rules; the best one also dealt paired and mirrored)
+    bit for bit."""
     c = jet256()
     out = []
-    for seg in (0, 1, 7, -1, -2):
-        print(f"full size: fp16 jet 256x512x256, 60 steps, segments {seg or 'timed'}", flush=True)
+    for seg in (0, 1, 7, -1, -2, -4, -5):
+        print(f"full size: fp16 jet 256x512x256, 100 steps, segments {seg or 'timed'}", flush=True)
         s = m.Solver(c, storage="fp16")
         s.set_launch_segments(seg)
         s.init_jet()
-        s.step(60)
+        s.step(100)  # the timing takes two plain steps per candidate shape (about 32 here)
         f = s.fields()
         out.append(({k: f[k] for k in ("rhor", "rhob", "ux", "uy", "uz")}, s.radial()["count"], s.launch_info()))
         s.close()
     print(f"launch shapes: {[o[2] for o in out]}", flush=True)
     assert out[0][2]["tuned"] and not out[1][2]["tuned"] and out[1][2]["segments"] == 1
-    assert [o[2]["schedule"] for o in out[1:]] == ["uniform", "uniform", "banded", "banded"]
+    assert [o[2]["schedule"] for o in out[1:]] == ["uniform", "uniform", "banded", "banded", "banded-paired",
+                                                   "banded-mirrored"]
+    assert out[0][2]["schedule"] in ("uniform", "banded", "banded-paired", "banded-mirrored")
     for fa, ca, _ in out[1:]:
         assert all(np.array_equal(out[0][0][k], fa[k]) for k in fa)
         assert np.array_equal(out[0][1], ca)

@@ -27,7 +27,7 @@ def dump(tmp_path_factory):
             if not f:
                 continue
             if f[0] == "cand":
-                cands.append({"bands": int(f[1]), "n": int(f[2]), "span": float(f[3]), "units": []})
+                cands.append({"bands": int(f[1]), "n": int(f[2]), "span": float(f[3]), "deal": int(f[4]), "units": []})
             elif f[0] == "th":
                 cands[-1]["th"] = [int(v) for v in f[1:]]
             elif f[0] == "u":
@@ -46,7 +46,10 @@ SHAPES = [(172, 254, 32, 1.9), (256, 254, 32, 1.5), (44, 126, 32, 1.9), (22, 500
 @pytest.mark.parametrize("ntiles,iz,cus,fill", SHAPES)
 def test_banded_schedules_cover_every_tile_plane_once(dump, ntiles, iz, cus, fill):
     cands = dump(ntiles, iz, cus, fill)
-    assert 1 <= len(cands) <= 3
+    # up to three band rules dealt in contiguous shares, then the best one paired and mirrored
+    assert 3 <= len(cands) <= 5
+    assert [c["deal"] for c in cands] == [0] * (len(cands) - 2) + [1, 2]
+    assert cands[-1]["th"] == cands[-2]["th"] == cands[0]["th"]
     for c in cands:
         u = c["units"]
         assert len(u) == c["n"] and len(u) % XCDS == 0
@@ -76,16 +79,22 @@ def test_banded_xcd_shares_and_band_order(dump, ntiles, iz, cus, fill):
             real = [v for v in mine if v[2] >= v[1]]
             # padding only after the XCD's last unit
             assert mine[:len(real)] == real
-            # an equal contiguous share of the tile-major tile-planes
-            s0, s1 = W * k // XCDS, W * (k + 1) // XCDS
+            # contiguous: an equal share of the tile-major tile-planes; paired and
+            # mirrored: two of 16 equal chunks, k and k+8, or k and 15-k
+            if c["deal"] == 0:
+                want = list(range(W * k // XCDS, W * (k + 1) // XCDS))
+            else:
+                other = k + XCDS if c["deal"] == 1 else 2 * XCDS - 1 - k
+                want = [i for ch in (k, other) for i in range(W * ch // (2 * XCDS), W * (ch + 1) // (2 * XCDS))]
             idx = sorted(t * iz + z - 1 for t, zb, ze in real for z in range(zb, ze + 1))
-            assert idx == list(range(s0, s1))
+            assert idx == sorted(want)
             # band by band, tiles in order within a band; a unit lies within one band
             band = [int(np.searchsorted(edges, ze, side="left")) - 1 for _, _, ze in real]
             for (t, zb, ze), j in zip(real, band):
                 assert edges[j] < zb <= ze <= edges[j + 1]
-            order = [(j, t) for (t, _, _), j in zip(real, band)]
-            assert order == sorted(order) and len(set(order)) == len(order)
+            # (a tile may hold two units of one band where an XCD's two chunks meet in it)
+            order = [(j, t, zb) for (t, zb, _), j in zip(real, band)]
+            assert order == sorted(order)
 
 
 @pytest.mark.parametrize("ntiles,iz,cus,fill", SHAPES)
