@@ -140,8 +140,8 @@ __device__ __forceinline__ void fast_load(const DevParams &P, const void *__rest
 // flight together; fast_load's per-direction branches waited for the loads of
 // each branch in turn (about 17 us per plane at 256x512x256, where a plane of
 // the other rows takes 9.5), and the 16-bit step kernels spill 10 to 20 fewer
-// VGPRs with it (the jet step 4.5% faster).  The FP32 step, whose boundary
-// tiles were not its slowest, ran 2.5% slower with it and keeps fast_load.
+// VGPRs with it (the jet step 4.5% faster).  FP32: 2% faster when the FP32
+// step issues it before its split loads (k_fused), 2% slower after them.
 template <int ST, int SIDE>
 __device__ __forceinline__ Moments boundary_row_moments(const DevParams &P, const DevBufs &B, const FastSrc &s) {
     const float *bnd = SIDE > 0 ? B.in_src : B.out_src;
@@ -186,9 +186,7 @@ template <int ST>
 __device__ __forceinline__ Moments fast_moments(const DevParams &P0, const DevBufs &B, int x, int y, int z) {
     const DevParams P = fresh_strides(P0);
     const FastSrc s = make_src(P, x, y, z);
-    if constexpr (ST != MCG_STORE_FP32) {
-        if (s.yin != s.yout) return s.yin ? boundary_row_moments<ST, 1>(P, B, s) : boundary_row_moments<ST, -1>(P, B, s);
-    }
+    if (s.yin != s.yout) return s.yin ? boundary_row_moments<ST, 1>(P, B, s) : boundary_row_moments<ST, -1>(P, B, s);
     MomentAcc acc;
     unroll<0, Q>([&](auto Ic) {
         constexpr int I = decltype(Ic)::value;
@@ -462,15 +460,33 @@ void k_fused(DevParams P, DevBufs B, FusedGrid G) {
         int zln = zl;
         if (P.wrap_gradient && !P.wall) zln = wrapN(zl, P.NZ);
         const bool split = lead && split_xy && zln >= 1 && zln <= P.NZ - 2;
+        // FP32: a lane that cannot split its gather (positions off the
+        // interior, the jet's boundary rows) gathers before the collision, so
+        // its wave waits on the loads while the other waves of its SIMD
+        // collide (at 256x512x256 the jet step ran 3% faster, the rising
+        // bubble 4%).  Without walls it also goes before the split loads are
+        // issued, so their words are not live during it (the jet 2% faster
+        // again); with walls after them (the rising bubble ran 2.6% slower
+        // the other way).  With 16-bit storage the early gather spills 12 to
+        // 40 more VGPRs.  The ring slots of plane zl are read by no one before
+        // the barrier.
+        // (the same block twice, one compiled per instance: as a shared lambda
+        // it ran both orders slower)
+        if constexpr (kEarlySerial && !EXT) {
+            if (lead && active && !split) {
+                bool has_m;
+                const float ph = gather_pos<ST>(P, B, gx, gy, zl, nx2, has_m);
+                if (mrow >= 0) {
+                    float *ms = mring + (size_t((zl % 3) * TY + mrow) * NMOM) * 64 + tx;
+                    const float v[NMOM] = {nx2.rr, nx2.rb, nx2.ux, nx2.uy, nx2.uz, nx2.pxx,
+                                           nx2.pxy, nx2.pyy, nx2.pyz, nx2.pzz, nx2.pxz};
+                    for (int k = 0; k < NMOM; ++k) ms[k * 64] = v[k];
+                }
+                phs[zl & 3][lr][lc] = ph;
+            }
+        }
         if (split) raw_issue<ST>(P, B, gxn, gyn, zln, raw);
-        if constexpr (kEarlySerial) {
-            // FP32: a lane that cannot split its gather (positions off the
-            // interior, the jet's boundary rows) gathers before the collision,
-            // so its wave waits on the loads while the other waves of its SIMD
-            // collide (at 256x512x256 the jet step ran 3% faster, the rising
-            // bubble 4%).  With 16-bit storage the same order spills 12 to 40
-            // more VGPRs.  The ring slots of plane zl are read by no one
-            // before the barrier.
+        if constexpr (kEarlySerial && EXT) {
             if (lead && active && !split) {
                 bool has_m;
                 const float ph = gather_pos<ST>(P, B, gx, gy, zl, nx2, has_m);
