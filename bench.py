@@ -95,11 +95,18 @@ def copy_probe(nbytes: int, reps: int = 3) -> None:
         hip.hipFree(b)
 
 
-def settle_launch(s, limit: int = 200) -> int:
-    """Untimed steps until the solver's launch-shape timing (about 2 steps per
-    candidate launch shape, on steps without statistics) has chosen; returns them."""
+def settle_launch(s, limit: int = 1000) -> int:
+    """Untimed steps until the solver's launch-shape timings have chosen: the
+    plain steps' (about 2 steps per candidate shape) and, where the case samples
+    statistics, the statistics steps' (one statistics step per candidate, then
+    12 more); returns the steps taken."""
+    sampled = s.case.NSTATS_START_STEP <= s.step_count + limit
+
+    def settled(info):
+        return info["tuned"] and (info["stats"]["tuned"] or not sampled)
+
     n = 0
-    while not s.launch_info()["tuned"] and n < limit:
+    while not settled(s.launch_info()) and n < limit:
         s.step(4)
         n += 4
     s.synchronize()

@@ -146,8 +146,8 @@ int mcg_set_kernel(mcg_solver *s, int kernel);
  * segments of equal length (uniform shapes) or, per XCD, into z-bands of
  * decreasing thickness (banded schedules).  segments = 0 (the default): the
  * solver's first plain steps (no statistics) try the candidate shapes in turn,
- * each timed with HIP events and waited for, and keep the fastest of two
- * passes.  segments > 0: the uniform shape nearest to that many segments,
+ * each timed with HIP events and waited for, two passes over all of them and
+ * three more over the four fastest, and keep the fastest.  segments > 0: the uniform shape nearest to that many segments,
  * fixed, with no timing.  segments = -k: the k-th banded schedule (1 = the
  * shortest estimated span), fixed; MCG_E_INVALID if there is none.  The
  * results do not depend on the launch shape. */
@@ -159,6 +159,15 @@ int mcg_set_launch_segments(mcg_solver *s, int segments);
  * dealt to the XCDs in contiguous shares, in chunk pairs k and k+8, or in
  * chunk pairs k and 15-k).  Any pointer may be NULL. */
 int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned, int32_t *banded);
+
+/* The launch shape of statistics steps (an extension, as above): they run
+ * another instance of the step kernel (8-row tiles), whose best shapes differ
+ * from the plain step's.  With segments = 0 the first statistics steps time
+ * uniform z segmentations (one pass each, three more over the four fastest)
+ * and keep the fastest; until then, and with a fixed shape, they take the
+ * plain step's shape re-tiled.  Reported: z segments, workgroups, chosen by
+ * the timing or not.  Any pointer may be NULL. */
+int mcg_stats_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned);
 
 /* ---- case API: initial conditions ------------------------------------- */
 

@@ -84,6 +84,52 @@ int scratch_resolve(int device) {
 
 }  // namespace
 
+// Timed choice among launch shapes of the step kernel: `passes` passes over
+// every candidate, then kRefinePasses more over the kRefineTop fastest (the
+// best shapes of a grid lie within about 1% of each other, a margin two
+// samples do not resolve), keeping the fastest time seen.  Each timed launch
+// runs between two HIP events and is waited for.
+struct ShapeTuner {
+    static constexpr int kRefineTop = 4, kRefinePasses = 3;
+    std::vector<mcg::FusedGrid> cand;
+    int passes = 2;
+    std::vector<float> ms;
+    std::vector<size_t> top;
+    size_t next = 0;
+    bool on = true, done = false;
+    mcg::FusedGrid chosen{};
+    bool active() const { return on && !done && cand.size() > 1; }
+    void reset(bool enable) {
+        ms.clear();
+        top.clear();
+        next = 0;
+        done = false;
+        on = enable;
+    }
+    size_t first() const { return cand.size() * size_t(passes); }
+    size_t total() const { return first() + std::min(cand.size(), size_t(kRefineTop)) * size_t(kRefinePasses); }
+    // the candidate of the next timed launch
+    size_t pick() {
+        const size_t n = cand.size();
+        if (ms.size() != n) ms.assign(n, 1e30f);
+        if (next == first()) {
+            std::vector<size_t> order(n);
+            for (size_t i = 0; i < n; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return ms[a] < ms[b]; });
+            order.resize(std::min(n, size_t(kRefineTop)));
+            top = order;
+        }
+        return next < first() ? next % n : top[(next - first()) % top.size()];
+    }
+    void record(size_t k, float t) {
+        ms[k] = std::min(ms[k], t);
+        if (++next >= total()) {
+            chosen = cand[size_t(std::min_element(ms.begin(), ms.end()) - ms.begin())];
+            done = true;
+        }
+    }
+};
+
 struct mcg_solver {
     mcg_case c{};
     DevParams P{};
@@ -110,15 +156,17 @@ struct mcg_solver {
     size_t staging_bytes = 0;
 
     // launch shape of the fused step: the static choice, then the fastest of the
-    // candidate z segmentations timed on the first plain steps (two passes)
+    // candidate shapes timed on the first plain steps (uniform z segmentations
+    // and banded schedules).  Statistics steps run another instance with its
+    // own best shapes (8-row tiles, moments in registers) and time uniform
+    // segmentations of their own on the first statistics steps; until then
+    // they take the plain steps' shape.
     FusedGrid grid{};
-    std::vector<FusedGrid> tune_cand;
+    ShapeTuner tune_plain, tune_stats;
     FusedUnit *units = nullptr;  // unit tables of the banded candidates (one allocation)
-    std::vector<float> tune_ms;
-    int tune_next = 0;
-    bool tune_on = true, tuned = false;
     hipEvent_t tune_ev[2] = {nullptr, nullptr};
-    static constexpr int kTunePasses = 2;
+    // the shape a statistics step launches with
+    FusedGrid stats_grid() const { return tune_stats.done ? tune_stats.chosen : fused_grid_for_instance(P, grid, kStatsTY); }
 
     int cur = 0;
     int64_t step = 0;
@@ -383,35 +431,34 @@ int do_init(mcg_solver *s, int kind) {
     return MCG_OK;
 }
 
-// One fused step.  Until the launch shape is tuned, plain steps (no
-// statistics, no recorded moments) run the candidate segmentations in turn,
-// each between two events and waited for, and the fastest of two passes is
-// kept; every other step uses the current choice.
-int launch_step(mcg_solver *s, const DevBufs &B) {
-    const bool tuning = s->tune_on && !s->tuned && !B.do_stats && !B.mom_out && s->tune_cand.size() > 1;
-    if (!tuning) {
-        const int rc = launch_fused(s->P, B, s->stream, s->grid);
-        if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
-        return MCG_OK;
-    }
+// One timed launch of the tuner's next candidate.
+int timed_launch(mcg_solver *s, const DevBufs &B, ShapeTuner &T) {
     for (hipEvent_t &e : s->tune_ev)
         if (!e) HIP_TRY(hipEventCreate(&e));
-    const size_t n = s->tune_cand.size();
-    const size_t k = size_t(s->tune_next) % n;
+    const size_t k = T.pick();
     HIP_TRY(hipEventRecord(s->tune_ev[0], s->stream));
-    const int rc = launch_fused(s->P, B, s->stream, s->tune_cand[k]);
+    const int rc = launch_fused(s->P, B, s->stream, T.cand[k]);
     if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
     HIP_TRY(hipEventRecord(s->tune_ev[1], s->stream));
     HIP_TRY(hipEventSynchronize(s->tune_ev[1]));
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, s->tune_ev[0], s->tune_ev[1]));
-    if (s->tune_ms.size() != n) s->tune_ms.assign(n, 1e30f);
-    s->tune_ms[k] = std::min(s->tune_ms[k], ms);
-    if (++s->tune_next >= int(n) * mcg_solver::kTunePasses) {
-        const size_t best = size_t(std::min_element(s->tune_ms.begin(), s->tune_ms.end()) - s->tune_ms.begin());
-        s->grid = s->tune_cand[best];
-        s->tuned = true;
+    T.record(k, ms);
+    return MCG_OK;
+}
+
+// One fused step.  Until their launch shapes are chosen, plain steps (no
+// statistics, no recorded moments) and statistics steps time their
+// candidates in turn (ShapeTuner); every other step uses the current choice.
+int launch_step(mcg_solver *s, const DevBufs &B) {
+    if (B.do_stats && s->tune_stats.active()) return timed_launch(s, B, s->tune_stats);
+    if (!B.do_stats && !B.mom_out && s->tune_plain.active()) {
+        const int rc = timed_launch(s, B, s->tune_plain);
+        if (s->tune_plain.done) s->grid = s->tune_plain.chosen;
+        return rc;
     }
+    const int rc = launch_fused(s->P, B, s->stream, B.do_stats ? s->stats_grid() : s->grid);
+    if (rc != 0) return fail(MCG_E_HIP, std::string("fused step launch: ") + hipGetErrorString(hipError_t(rc)));
     return MCG_OK;
 }
 
@@ -601,7 +648,13 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
         return rc;
     }
     s->grid = fused_grid(s->P, s->num_cu);
-    s->tune_cand = fused_candidates(s->P);
+    s->tune_plain.cand = fused_candidates(s->P);
+    // statistics steps: uniform segmentations of 8-row tiles (at 256x512x256
+    // the FP32 statistics step took 3.44 ms with 13 segments, 3.55 with 4 and
+    // 3.77 with the 2-band schedule the plain step kept; one pass each, as
+    // they come one step in NSTATS_SAMPLE)
+    s->tune_stats.cand = fused_candidates_ty(s->P, kStatsTY, 32);
+    s->tune_stats.passes = 1;
     {
         // the banded schedules join the timed candidates
         const std::vector<BandedSchedule> banded = banded_candidates(s->P, s->num_cu);
@@ -625,7 +678,7 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
                 G.units = s->units + off;
                 G.deal = int(b.deal);
                 off += b.units.size();
-                s->tune_cand.push_back(G);
+                s->tune_plain.cand.push_back(G);
             }
         }
     }
@@ -712,12 +765,10 @@ int mcg_set_launch_segments(mcg_solver *s, int segments) {
     if (int rc = check_solver(s)) return rc;
     if (segments < 0) {
         int k = 0;
-        for (const FusedGrid &G : s->tune_cand) {
+        for (const FusedGrid &G : s->tune_plain.cand) {
             if (G.units && ++k == -segments) {
-                s->tuned = false;
-                s->tune_next = 0;
-                s->tune_ms.clear();
-                s->tune_on = false;
+                s->tune_plain.reset(false);
+                s->tune_stats.reset(false);
                 s->grid = G;
                 return MCG_OK;
             }
@@ -725,15 +776,13 @@ int mcg_set_launch_segments(mcg_solver *s, int segments) {
         return fail(MCG_E_INVALID, "no banded launch schedule " + std::to_string(-segments) + " (" + std::to_string(k) +
                                        " for this grid)");
     }
-    s->tuned = false;
-    s->tune_next = 0;
-    s->tune_ms.clear();
-    s->tune_on = segments == 0;
+    s->tune_plain.reset(segments == 0);
+    s->tune_stats.reset(segments == 0);
     s->grid = fused_grid(s->P, s->num_cu);
     if (segments > 0) {
         // the uniform candidate with the segment count nearest to the one asked for
-        const FusedGrid *best = &s->tune_cand.front();
-        for (const FusedGrid &G : s->tune_cand)
+        const FusedGrid *best = &s->tune_plain.cand.front();
+        for (const FusedGrid &G : s->tune_plain.cand)
             if (!G.units && std::abs(G.nzs - segments) < std::abs(best->nzs - segments)) best = &G;
         s->grid = *best;
     }
@@ -744,8 +793,17 @@ int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups,
     if (int rc = check_solver(s)) return rc;
     if (segments) *segments = s->grid.nzs;
     if (workgroups) *workgroups = s->grid.nwg;
-    if (tuned) *tuned = s->tuned ? 1 : 0;
+    if (tuned) *tuned = s->tune_plain.done ? 1 : 0;
     if (banded) *banded = s->grid.units ? 1 + s->grid.deal : 0;
+    return MCG_OK;
+}
+
+int mcg_stats_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned) {
+    if (int rc = check_solver(s)) return rc;
+    const FusedGrid G = s->stats_grid();
+    if (segments) *segments = G.nzs;
+    if (workgroups) *workgroups = G.nwg;
+    if (tuned) *tuned = s->tune_stats.done ? 1 : 0;
     return MCG_OK;
 }
 

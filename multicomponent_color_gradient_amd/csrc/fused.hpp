@@ -607,15 +607,17 @@ inline FusedGrid fused_grid_for(const DevParams &P, int zs, int ty) {
 // padding of the row blocks (32 to 320 words).  The solver times every
 // candidate on its first steps and keeps the fastest; the results do not
 // depend on the segmentation.
-inline std::vector<FusedGrid> fused_candidates(const DevParams &P) {
-    const int ty = fused_plain_ty(P.storage);
+inline std::vector<FusedGrid> fused_candidates_ty(const DevParams &P, int ty, int max_segments) {
     std::vector<FusedGrid> all;
-    const int maxseg = std::max(1, std::min(64, P.IZ / 8));
+    const int maxseg = std::max(1, std::min(max_segments, P.IZ / 8));
     for (int zs = 1; zs <= maxseg; ++zs) {
         const FusedGrid G = fused_grid_for(P, zs, ty);
         if (all.empty() || all.back().nzs != G.nzs) all.push_back(G);
     }
     return all;
+}
+inline std::vector<FusedGrid> fused_candidates(const DevParams &P) {
+    return fused_candidates_ty(P, fused_plain_ty(P.storage), 64);
 }
 
 // The banded candidates of a launch: the kBandedCandidates band rules of
@@ -660,11 +662,17 @@ inline bool fused_has_stats(const DevParams &P, size_t lds_per_block) {
     return fused_stats_lds(P) + fused_static_lds() <= lds_per_block;
 }
 
-// one instance on grid G, re-tiled to the instance's rows with G's segment count
+// grid G for an instance of ty rows: G itself, or uniform segments of ty-row
+// tiles with G's segment (or band) count
+inline FusedGrid fused_grid_for_instance(const DevParams &P, const FusedGrid &G, int ty) {
+    return G.ty == ty ? G : fused_grid_for(P, G.nzs, ty);
+}
+
+// one instance on grid G, re-tiled to the instance's rows if they differ
 template <bool STATS, int ST, bool MOM, bool EXT>
 inline void launch_fused_one(const DevParams &P, const DevBufs &B, hipStream_t stream, const FusedGrid &G, size_t lds) {
     using Shape = FusedShape<ST, STATS>;
-    const FusedGrid Gi = G.ty == Shape::TY ? G : fused_grid_for(P, G.nzs, Shape::TY);
+    const FusedGrid Gi = fused_grid_for_instance(P, G, Shape::TY);
     hipLaunchKernelGGL((k_fused<STATS, ST, MOM, EXT>), dim3(Gi.nwg), dim3(64, Shape::kWaves, 1), lds, stream, P, B, Gi);
 }
 

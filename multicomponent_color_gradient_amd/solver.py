@@ -77,8 +77,11 @@ class Solver:
         seg, wg, tuned, banded = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         check(self.lib.mcg_launch_info(self._h, ctypes.byref(seg), ctypes.byref(wg), ctypes.byref(tuned),
                                        ctypes.byref(banded)))
+        sseg, swg, stuned = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(self.lib.mcg_stats_launch_info(self._h, ctypes.byref(sseg), ctypes.byref(swg), ctypes.byref(stuned)))
         return {"segments": seg.value, "workgroups": wg.value, "tuned": bool(tuned.value),
-                "schedule": ("uniform", "banded", "banded-paired", "banded-mirrored")[banded.value]}
+                "schedule": ("uniform", "banded", "banded-paired", "banded-mirrored")[banded.value],
+                "stats": {"segments": sseg.value, "workgroups": swg.value, "tuned": bool(stuned.value)}}
 
     def init_jet(self) -> None:
         check(self.lib.mcg_init_jet(self._h))
