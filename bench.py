@@ -312,8 +312,10 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=200)
     ap.add_argument("--segments", type=int, default=0,
-                    help="z segments of the step kernel (0: the solver times the candidates on its first steps, "
-                         "the default; n: about n, fixed, e.g. to profile the shape a default run chose)")
+                    help="launch shape of the step kernel (0: the solver times the candidates on its first steps, "
+                         "the default; n > 0: about n uniform z segments, fixed; -k: the k-th banded schedule, "
+                         "fixed; config.launch.shape of a default run is the value that fixes the shape it chose, "
+                         "e.g. to profile it)")
     ap.add_argument("--no-ceiling", action="store_true",
                     help="skip the short read/write/copy bandwidth probe reported beside the step")
     ap.add_argument("--ceiling", action="store_true",

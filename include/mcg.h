@@ -157,8 +157,11 @@ int mcg_set_launch_segments(mcg_solver *s, int segments);
  * schedule), workgroups per step, whether the timing has chosen it, and
  * whether it is banded (0: uniform; 1, 2, 3: banded with the tile-planes
  * dealt to the XCDs in contiguous shares, in chunk pairs k and k+8, or in
- * chunk pairs k and 15-k).  Any pointer may be NULL. */
-int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned, int32_t *banded);
+ * chunk pairs k and 15-k), and the mcg_set_launch_segments argument that
+ * fixes this shape (n segments, or -k for the k-th banded schedule).  Any
+ * pointer may be NULL. */
+int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned, int32_t *banded,
+                    int32_t *shape);
 
 /* The launch shape of statistics steps (an extension, as above): they run
  * another instance of the step kernel (8-row tiles), whose best shapes differ

@@ -134,7 +134,8 @@ SIGNATURES = [
     ("mcg_global_sums", c_int, [_PSOLVER, _D]),
     ("mcg_global_sums_noalloc", c_int, [_PSOLVER, c_void_p]),
     ("mcg_set_launch_segments", c_int, [_PSOLVER, c_int]),
-    ("mcg_launch_info", c_int, [_PSOLVER, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+    ("mcg_launch_info", c_int,
+     [_PSOLVER, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     ("mcg_stats_launch_info", c_int, [_PSOLVER, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     ("mcg_hbm_probe", c_int, [c_int, c_int, c_int, c_int, c_uint64, c_int, _D]),
     ("mcg_check_health", c_int, [_PSOLVER]),

@@ -789,12 +789,20 @@ int mcg_set_launch_segments(mcg_solver *s, int segments) {
     return MCG_OK;
 }
 
-int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned, int32_t *banded) {
+int mcg_launch_info(const mcg_solver *s, int32_t *segments, int32_t *workgroups, int32_t *tuned, int32_t *banded,
+                    int32_t *shape) {
     if (int rc = check_solver(s)) return rc;
     if (segments) *segments = s->grid.nzs;
     if (workgroups) *workgroups = s->grid.nwg;
     if (tuned) *tuned = s->tune_plain.done ? 1 : 0;
     if (banded) *banded = s->grid.units ? 1 + s->grid.deal : 0;
+    if (shape) {
+        // the mcg_set_launch_segments argument that fixes this shape
+        *shape = s->grid.nzs;
+        int k = 0;
+        for (const FusedGrid &G : s->tune_plain.cand)
+            if (G.units && (++k, G.units == s->grid.units)) *shape = -k;
+    }
     return MCG_OK;
 }
 

@@ -73,14 +73,16 @@ class Solver:
 
     def launch_info(self) -> dict:
         """The step kernel's launch shape: z segments (or z-bands of a banded
-        schedule), workgroups per step, tuned or not, banded or uniform."""
-        seg, wg, tuned, banded = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        schedule), workgroups per step, tuned or not, banded or uniform, the
+        set_launch_segments argument that fixes it, and the statistics steps' shape."""
+        seg, wg, tuned, banded, shape = (ctypes.c_int32() for _ in range(5))
         check(self.lib.mcg_launch_info(self._h, ctypes.byref(seg), ctypes.byref(wg), ctypes.byref(tuned),
-                                       ctypes.byref(banded)))
+                                       ctypes.byref(banded), ctypes.byref(shape)))
         sseg, swg, stuned = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         check(self.lib.mcg_stats_launch_info(self._h, ctypes.byref(sseg), ctypes.byref(swg), ctypes.byref(stuned)))
         return {"segments": seg.value, "workgroups": wg.value, "tuned": bool(tuned.value),
                 "schedule": ("uniform", "banded", "banded-paired", "banded-mirrored")[banded.value],
+                "shape": shape.value,
                 "stats": {"segments": sseg.value, "workgroups": swg.value, "tuned": bool(stuned.value)}}
 
     def init_jet(self) -> None:

@@ -122,6 +122,7 @@ def test_launch_tuning_keeps_results_bitwise():
     assert [o[2]["schedule"] for o in out[1:]] == ["uniform", "uniform", "banded", "banded", "banded-paired",
                                                    "banded-mirrored"]
     assert out[0][2]["schedule"] in ("uniform", "banded", "banded-paired", "banded-mirrored")
+    assert [o[2]["shape"] for o in out[1:]] == [1, 7, -1, -2, -4, -5]
     for fa, ca, _ in out[1:]:
         assert all(np.array_equal(out[0][0][k], fa[k]) for k in fa)
         assert np.array_equal(out[0][1], ca)
