@@ -281,6 +281,7 @@ struct mcg_solver {
         B.stats_fx = stats_fx;
         B.count = count;
         B.do_stats = do_stats ? 1 : 0;
+        B.stats_ring = fused_stats_ring(P, fused_lds) ? 1 : 0;
         B.step = (unsigned long long)(step);
         return B;
     }
@@ -683,11 +684,12 @@ int mcg_create(const mcg_case *c, int storage, void *stream, mcg_solver **out) {
         }
     }
     {
-        // histograms larger than the default 64 KiB need the kernel's dynamic-LDS limit raised
+        // a workgroup's LDS above the default 64 KiB (the histogram beside the
+        // psi and moment rings) needs the kernels' dynamic-LDS limit raised
         int lds = 0;
-        const size_t need = fused_stats_lds(s->P) + fused_static_lds();
+        const size_t need = fused_stats_lds(s->P) + fused_static_lds(false);
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) == hipSuccess &&
-            size_t(lds) >= need && need > 65536) {
+            size_t(lds) >= need) {
             int rc = 0;
             switch (storage) {
                 case MCG_STORE_FP16: rc = fused_allow_lds<MCG_STORE_FP16>(fused_stats_lds(s->P)); break;

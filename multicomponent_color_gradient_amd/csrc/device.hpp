@@ -80,6 +80,7 @@ struct DevBufs {
     unsigned long long *stats_fx;  // 7 planes of NY*NR (uy, uy2, ur, ur2, uruy, phi, phi2): this step's sums, fixed point
     unsigned long long *count;  // NY*NR
     int do_stats;
+    int stats_ring;  // statistics steps: the instance with the LDS moment ring (the histogram fits beside it)
     unsigned long long step;    // index of the step being computed (0 for the first)
 };
 

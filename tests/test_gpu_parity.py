@@ -227,15 +227,20 @@ def test_radial_statistics_kernels(mcg, kernel):
     assert np.all(np.abs(g["sum_uy"] - r["sum_uy"]) <= 1e-6 * r["count"] + 1e-12)
 
 
-def test_statistics_histogram_above_64k_lds(mcg):
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+@pytest.mark.parametrize("nx", [256, 344])
+def test_statistics_histogram_above_64k_lds(mcg, nx, storage):
     """NX=256 gives 128 radial bins: the fused kernel's LDS histogram (8 rows x
-    128 bins x 64 B) exceeds 64 KiB; its statistics must equal the two-pass
-    path's (separate moments + k_radial) bit for bit, and so must the populations."""
-    case = mcg.jet_case(256, 14, 12, jet_radius=4.0, Re=500.0)
+    128 bins x 64 B = 64 KiB) sits beside the psi ring and the moment ring (the
+    statistics instance with RING). NX=344 gives 172 bins (88 KiB), too many for
+    the moment ring as well: the instance with the moments in registers. Either
+    way the statistics must equal the two-pass path's (separate moments +
+    k_radial) bit for bit, and so must the populations."""
+    case = mcg.jet_case(nx, 14, 12, jet_radius=4.0, Re=500.0)
     case.NSTATS_START_STEP = 0
     case.NSTATS_SAMPLE = 2
-    a = mcg.Solver(case, kernel="fused")
-    b = mcg.Solver(case, kernel="twopass")
+    a = mcg.Solver(case, kernel="fused", storage=storage)
+    b = mcg.Solver(case, kernel="twopass", storage=storage)
     a.init_jet()
     b.init_jet()
     a.step(9)
