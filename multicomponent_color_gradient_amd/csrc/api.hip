@@ -158,7 +158,7 @@ struct mcg_solver {
     // launch shape of the fused step: the static choice, then the fastest of the
     // candidate shapes timed on the first plain steps (uniform z segmentations
     // and banded schedules).  Statistics steps run another instance with its
-    // own best shapes (8-row tiles, moments in registers) and time uniform
+    // own best shapes (8-row tiles, the histogram in LDS) and time uniform
     // segmentations of their own on the first statistics steps; until then
     // they take the plain steps' shape.
     FusedGrid grid{};
